@@ -1,0 +1,130 @@
+// common.hpp -- shared device/host helpers for the gfx950 kernels of
+// libgputils_hip.so.  Everything here is CDNA-specific: 64-wide wavefronts,
+// 16x16x4 MFMA tiles, LDS-resident per-matrix working sets.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "gputils_hip.h"
+
+namespace gputils {
+
+constexpr int kWave = 64;
+
+// ------------------------------------------------------------ error state --
+// Thread-local message of the last failing call; read by gputils_last_error().
+void set_error(const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what);
+
+#define GPUTILS_HIP(call)                                      \
+    do {                                                       \
+        hipError_t e_ = (call);                                \
+        if (e_ != hipSuccess) return ::gputils::hip_fail(e_, #call); \
+    } while (0)
+
+#define GPUTILS_ARG(cond, idx, msg)                                        \
+    do {                                                                   \
+        if (!(cond)) {                                                     \
+            ::gputils::set_error("invalid argument %d: %s", (idx), (msg)); \
+            return -(idx);                                                 \
+        }                                                                  \
+    } while (0)
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Kernel launch epilogue: report launch-configuration errors.
+inline int launch_check(const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, what);
+    return GPUTILS_OK;
+}
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Grid-stride loops must never exceed the 2^31-1 blocks limit of x.
+inline unsigned grid_1d(int64_t work, int per_block, int64_t cap = 65536) {
+    int64_t g = ceil_div(work, per_block);
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+// -------------------------------------------------------------- MFMA ops --
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// 16x16x4 MFMA in float or double.  Operand layout (CDNA3/CDNA4 ISA):
+//   A (16x4):  lane l holds A[l % 16][l / 16]
+//   B (4x16):  lane l holds B[l / 16][l % 16]
+//   D (16x16): lane l, item r holds D[4 * (l / 16) + r][l % 16]
+template <typename T>
+struct Mfma16;
+
+template <>
+struct Mfma16<float> {
+    using acc_t = f32x4;
+    __device__ static inline acc_t zero() { return acc_t{0.f, 0.f, 0.f, 0.f}; }
+    __device__ static inline acc_t fma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+template <>
+struct Mfma16<double> {
+    using acc_t = f64x4;
+    __device__ static inline acc_t zero() { return acc_t{0.0, 0.0, 0.0, 0.0}; }
+    __device__ static inline acc_t fma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+
+// ------------------------------------------------------- lane exchanges --
+__device__ inline float readlane(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ inline double readlane(double v, int lane) {
+    long long b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+    int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ inline int readlane(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+
+// Wave-wide sum / max through xor shuffles (ds_swizzle/bpermute).
+template <typename T>
+__device__ inline T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+template <typename T>
+__device__ inline T wave_max(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        T o = __shfl_xor(v, off, kWave);
+        v = (o > v) ? o : v;
+    }
+    return v;
+}
+template <typename T>
+__device__ inline T wave_min(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        T o = __shfl_xor(v, off, kWave);
+        v = (o < v) ? o : v;
+    }
+    return v;
+}
+
+template <typename T>
+__device__ inline T dev_sqrt(T x);
+template <>
+__device__ inline float dev_sqrt<float>(float x) { return __builtin_sqrtf(x); }
+template <>
+__device__ inline double dev_sqrt<double>(double x) { return __builtin_sqrt(x); }
+
+}  // namespace gputils

@@ -1,0 +1,310 @@
+// gemm.hip -- batched C_i <- alpha A_i B_i + beta C_i for gfx950.
+//
+// Replaces cublas?gemmBatched / ?gemm behind DTensor::addAB and operator*
+// (include/tensor.cuh:1286-1338, 645-650).
+//
+// Tiled path: one 256-thread workgroup computes a 64x64 tile of one C_i.
+// K is staged through LDS in chunks (32 for float, 16 for double) with the
+// k index contiguous for both operands (A tile stored transposed).  Each
+// wavefront owns a 32x32 quadrant = 2x2 MFMA tiles of 16x16:
+//   float, FP32 mode   v_mfma_f32_16x16x4_f32      (exact fp32 products)
+//   float, BF16X3 mode v_mfma_f32_16x16x32_bf16 x3: a = ah + al, b = bh + bl
+//                      (bf16 terms), C += ah bh + ah bl + al bh, fp32 acc.
+//   double             v_mfma_f64_16x16x4_f64
+// The C tile is staged through LDS and written with 16-byte stores.
+//
+// Per-matrix path (C aliasing B, or very narrow C): one workgroup per
+// matrix stages all of B_i in LDS before any C_i element is written, which
+// makes the in-place product of Nullspace::project (tensor.cuh:2084) safe.
+#include "common.hpp"
+
+namespace gputils {
+namespace gemm {
+
+constexpr int kThreads = 256;
+constexpr int kTM = 64, kTN = 64;
+
+template <typename T>
+struct Cfg;
+template <>
+struct Cfg<float> {
+    static constexpr int TK = 32, S = 36;  // k-chunk, LDS row stride (16-byte aligned rows)
+};
+template <>
+struct Cfg<double> {
+    static constexpr int TK = 16, S = 17;
+};
+
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ inline void split_bf16(const float *v, bf16x8 &hi, bf16x8 &lo) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const __bf16 h = (__bf16)v[e];
+        hi[e] = h;
+        lo[e] = (__bf16)(v[e] - (float)h);
+    }
+}
+
+// Stage A[i0:i0+64, k0:k0+TK] as As[i][k] and B[k0:k0+TK, j0:j0+64] as
+// Bs[j][k] (both with row stride S), zero outside the matrix.
+template <typename T>
+__device__ void stage(const T *a, int64_t lda, const T *b, int64_t ldb, int m, int n, int k, int i0,
+                      int j0, int k0, T *As, T *Bs) {
+    constexpr int TK = Cfg<T>::TK, S = Cfg<T>::S;
+    // A: consecutive threads walk down a column (contiguous in memory)
+    for (int e = threadIdx.x; e < kTM * TK; e += kThreads) {
+        const int i = e % kTM, kk = e / kTM;
+        const int gi = i0 + i, gk = k0 + kk;
+        As[i * S + kk] = (gi < m && gk < k) ? a[gi + (int64_t)gk * lda] : T(0);
+    }
+    // B: consecutive threads walk down a column of B (k contiguous)
+    for (int e = threadIdx.x; e < kTN * TK; e += kThreads) {
+        const int kk = e % TK, j = e / TK;
+        const int gk = k0 + kk, gj = j0 + j;
+        Bs[j * S + kk] = (gk < k && gj < n) ? b[gk + (int64_t)gj * ldb] : T(0);
+    }
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(kThreads) void gemm_tiled_kernel(int m, int n, int k, T alpha,
+                                                              const T *__restrict__ A, int64_t lda,
+                                                              int64_t strideA,
+                                                              const T *__restrict__ B, int64_t ldb,
+                                                              int64_t strideB, T beta, T *C,
+                                                              int64_t ldc, int64_t strideC,
+                                                              int tiles_m, int64_t batch) {
+    constexpr int TK = Cfg<T>::TK, S = Cfg<T>::S;
+    __shared__ __attribute__((aligned(16))) T smem[2 * kTM * S > kTM * (kTN + 1) ? 2 * kTM * S : kTM * (kTN + 1)];
+    T *As = smem, *Bs = smem + kTM * S;
+    using M = Mfma16<T>;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = 32 * (wave & 1), wn = 32 * (wave >> 1);
+    const int i0 = (blockIdx.x % tiles_m) * kTM, j0 = (blockIdx.x / tiles_m) * kTN;
+    const int64_t bidx = blockIdx.y + (int64_t)blockIdx.z * 65535;
+    if (bidx >= batch) return;  // uniform per workgroup
+    const T *a = A + bidx * strideA;
+    const T *b = B + bidx * strideB;
+    T *c = C + bidx * strideC;
+
+    typename M::acc_t acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] = M::zero();
+
+    for (int k0 = 0; k0 < k; k0 += TK) {
+        __syncthreads();
+        stage<T>(a, lda, b, ldb, m, n, k, i0, j0, k0, As, Bs);
+        __syncthreads();
+        if constexpr (MODE == GPUTILS_GEMM_BF16X3) {
+            // one 16x16x32 step per tile: lane holds 8 consecutive k
+            const int r = lane & 15, g = lane >> 4;
+            bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+            for (int x = 0; x < 2; ++x) {
+                const float4 *pa = reinterpret_cast<const float4 *>(As + (wm + 16 * x + r) * S + 8 * g);
+                const float4 *pb = reinterpret_cast<const float4 *>(Bs + (wn + 16 * x + r) * S + 8 * g);
+                float va[8], vb[8];
+                const float4 a0 = pa[0], a1 = pa[1], b0 = pb[0], b1 = pb[1];
+                va[0] = a0.x; va[1] = a0.y; va[2] = a0.z; va[3] = a0.w;
+                va[4] = a1.x; va[5] = a1.y; va[6] = a1.z; va[7] = a1.w;
+                vb[0] = b0.x; vb[1] = b0.y; vb[2] = b0.z; vb[3] = b0.w;
+                vb[4] = b1.x; vb[5] = b1.y; vb[6] = b1.z; vb[7] = b1.w;
+                split_bf16(va, ah[x], al[x]);
+                split_bf16(vb, bh[x], bl[x]);
+            }
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y) {
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[x], bh[y], acc[x][y], 0, 0, 0);
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bl[y], acc[x][y], 0, 0, 0);
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bh[y], acc[x][y], 0, 0, 0);
+                }
+        } else {
+            const int r = lane & 15, g = lane >> 4;
+#pragma unroll
+            for (int kk = 0; kk < TK; kk += 4) {
+                T av[2], bv[2];
+#pragma unroll
+                for (int x = 0; x < 2; ++x) {
+                    av[x] = As[(wm + 16 * x + r) * S + kk + g];
+                    bv[x] = Bs[(wn + 16 * x + r) * S + kk + g];
+                }
+#pragma unroll
+                for (int x = 0; x < 2; ++x)
+#pragma unroll
+                    for (int y = 0; y < 2; ++y) acc[x][y] = M::fma(av[x], bv[y], acc[x][y]);
+            }
+        }
+    }
+    // stage the 64x64 tile column-major (ld 65) and write coalesced
+    __syncthreads();
+    T *Cs = smem;
+    {
+        const int g = lane >> 4, col = lane & 15;
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    Cs[(wm + 16 * x + 4 * g + r) + (wn + 16 * y + col) * (kTM + 1)] = acc[x][y][r];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kTM * kTN; e += kThreads) {
+        const int i = e % kTM, j = e / kTM;
+        const int gi = i0 + i, gj = j0 + j;
+        if (gi < m && gj < n) {
+            T v = alpha * Cs[i + j * (kTM + 1)];
+            T *dst = c + gi + (int64_t)gj * ldc;
+            if (beta != T(0)) v += beta * *dst;
+            *dst = v;
+        }
+    }
+}
+
+// Per-matrix kernel: stage B_i (k x n) in LDS, then each thread group
+// computes C elements with a split-K dot product reduced through LDS.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void gemm_permatrix_kernel(int m, int n, int k, T alpha,
+                                                                  const T *__restrict__ A,
+                                                                  int64_t lda, int64_t strideA,
+                                                                  const T *B, int64_t ldb,
+                                                                  int64_t strideB, T beta, T *C,
+                                                                  int64_t ldc, int64_t strideC,
+                                                                  int split, int64_t batch) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char dyn[];
+    T *Bs = reinterpret_cast<T *>(dyn);       // k * n
+    T *red = Bs + (int64_t)k * n;              // kThreads partials
+    const int64_t bidx = blockIdx.x + (int64_t)blockIdx.y * 65535;
+    if (bidx >= batch) return;  // uniform per workgroup
+    const T *a = A + bidx * strideA;
+    const T *b = B + bidx * strideB;
+    T *c = C + bidx * strideC;
+    for (int e = threadIdx.x; e < k * n; e += kThreads) {
+        const int p = e % k, j = e / k;
+        Bs[e] = b[p + (int64_t)j * ldb];
+    }
+    __syncthreads();
+    const int groups = kThreads / split;  // output elements per pass
+    const int s = threadIdx.x / groups, oi = threadIdx.x % groups;
+    const int total = m * n;
+    for (int base = 0; base < total; base += groups) {
+        const int o = base + oi;
+        T acc = T(0);
+        if (o < total) {
+            const int i = o % m, j = o / m;
+            for (int p = s; p < k; p += split) acc += a[i + (int64_t)p * lda] * Bs[p + j * k];
+        }
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        if (s == 0 && o < total) {
+            T sum = acc;
+            for (int q = 1; q < split; ++q) sum += red[q * groups + oi];
+            const int i = o % m, j = o / m;
+            T *dst = c + i + (int64_t)j * ldc;
+            T v = alpha * sum;
+            if (beta != T(0)) v += beta * *dst;
+            *dst = v;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, int MODE>
+int run(int64_t m, int64_t n, int64_t k, T alpha, const T *A, int64_t lda, int64_t strideA,
+        const T *B, int64_t ldb, int64_t strideB, T beta, T *C, int64_t ldc, int64_t strideC,
+        int64_t batch, hipStream_t st) {
+    const bool alias = (static_cast<const void *>(B) == static_cast<const void *>(C));
+    if (alias && (ldb != ldc || strideB != strideC)) {
+        set_error("gemm: C may alias B only with identical ld and stride");
+        return -10;
+    }
+    const bool narrow = (n <= 4 && m * n <= 4096);
+    if (alias || narrow) {
+        const size_t lds = sizeof(T) * ((size_t)k * n + kThreads);
+        if (lds > 64 * 1024) {
+            set_error("gemm: in-place product needs k*n*sizeof(T) <= 62 KiB");
+            return -10;
+        }
+        // split-K factor: power of two, at most 32, so that m*n*split <= kThreads
+        const int64_t mn = m * n;
+        int split = 1;
+        while (split < 32 && split * 2 <= k && mn * split * 2 <= kThreads) split *= 2;
+        const unsigned gx = (unsigned)(batch < 65535 ? batch : 65535);
+        const unsigned gy = (unsigned)ceil_div(batch, 65535);
+        hipLaunchKernelGGL((gemm_permatrix_kernel<T>), dim3(gx, gy), dim3(kThreads), lds, st, (int)m,
+                           (int)n, (int)k, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc,
+                           strideC, split, batch);
+        return launch_check("gemm_permatrix_kernel");
+    }
+    const int tiles_m = (int)ceil_div(m, kTM), tiles_n = (int)ceil_div(n, kTN);
+    const int64_t tiles = (int64_t)tiles_m * tiles_n;
+    if (tiles > 0x7fffffff) {
+        set_error("gemm: too many tiles");
+        return -1;
+    }
+    const unsigned gy = (unsigned)(batch < 65535 ? batch : 65535);
+    const unsigned gz = (unsigned)ceil_div(batch, 65535);
+    hipLaunchKernelGGL((gemm_tiled_kernel<T, MODE>), dim3((unsigned)tiles, gy, gz), dim3(kThreads), 0,
+                       st, (int)m, (int)n, (int)k, alpha, A, lda, strideA, B, ldb, strideB, beta, C,
+                       ldc, strideC, tiles_m, batch);
+    return launch_check("gemm_tiled_kernel");
+}
+
+}  // namespace gemm
+}  // namespace gputils
+
+using namespace gputils;
+
+extern "C" {
+
+int gputils_sgemm_strided_batched(int64_t m, int64_t n, int64_t k, float alpha, const float *A,
+                                  int64_t lda, int64_t strideA, const float *B, int64_t ldb,
+                                  int64_t strideB, float beta, float *C, int64_t ldc,
+                                  int64_t strideC, int64_t batch, int mode, void *stream) {
+    GPUTILS_ARG(m >= 0, 1, "m < 0");
+    GPUTILS_ARG(n >= 0, 2, "n < 0");
+    GPUTILS_ARG(k >= 0, 3, "k < 0");
+    GPUTILS_ARG(lda >= (m > 1 ? m : 1), 6, "lda < max(1,m)");
+    GPUTILS_ARG(ldb >= (k > 1 ? k : 1), 9, "ldb < max(1,k)");
+    GPUTILS_ARG(ldc >= (m > 1 ? m : 1), 13, "ldc < max(1,m)");
+    GPUTILS_ARG(batch >= 0 && batch <= 65535ll * 65535ll, 15, "batch out of range");
+    GPUTILS_ARG(mode == GPUTILS_GEMM_FP32 || mode == GPUTILS_GEMM_BF16X3, 16, "unknown mode");
+    GPUTILS_ARG(m <= 0x7fffffff && n <= 0x7fffffff && k <= 0x7fffffff, 1, "dimension too large");
+    if (m == 0 || n == 0 || batch == 0) return GPUTILS_OK;
+    GPUTILS_ARG(C != nullptr, 12, "C is NULL");
+    GPUTILS_ARG(k == 0 || A != nullptr, 5, "A is NULL");
+    GPUTILS_ARG(k == 0 || B != nullptr, 8, "B is NULL");
+    hipStream_t st = as_stream(stream);
+    if (mode == GPUTILS_GEMM_BF16X3 && !(B == C))
+        return gemm::run<float, GPUTILS_GEMM_BF16X3>(m, n, k, alpha, A, lda, strideA, B, ldb, strideB,
+                                                     beta, C, ldc, strideC, batch, st);
+    return gemm::run<float, GPUTILS_GEMM_FP32>(m, n, k, alpha, A, lda, strideA, B, ldb, strideB, beta,
+                                               C, ldc, strideC, batch, st);
+}
+
+int gputils_dgemm_strided_batched(int64_t m, int64_t n, int64_t k, double alpha, const double *A,
+                                  int64_t lda, int64_t strideA, const double *B, int64_t ldb,
+                                  int64_t strideB, double beta, double *C, int64_t ldc,
+                                  int64_t strideC, int64_t batch, void *stream) {
+    GPUTILS_ARG(m >= 0, 1, "m < 0");
+    GPUTILS_ARG(n >= 0, 2, "n < 0");
+    GPUTILS_ARG(k >= 0, 3, "k < 0");
+    GPUTILS_ARG(lda >= (m > 1 ? m : 1), 6, "lda < max(1,m)");
+    GPUTILS_ARG(ldb >= (k > 1 ? k : 1), 9, "ldb < max(1,k)");
+    GPUTILS_ARG(ldc >= (m > 1 ? m : 1), 13, "ldc < max(1,m)");
+    GPUTILS_ARG(batch >= 0 && batch <= 65535ll * 65535ll, 15, "batch out of range");
+    GPUTILS_ARG(m <= 0x7fffffff && n <= 0x7fffffff && k <= 0x7fffffff, 1, "dimension too large");
+    if (m == 0 || n == 0 || batch == 0) return GPUTILS_OK;
+    GPUTILS_ARG(C != nullptr, 12, "C is NULL");
+    GPUTILS_ARG(k == 0 || A != nullptr, 5, "A is NULL");
+    GPUTILS_ARG(k == 0 || B != nullptr, 8, "B is NULL");
+    return gemm::run<double, 0>(m, n, k, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc,
+                                strideC, batch, as_stream(stream));
+}
+
+}  // extern "C"

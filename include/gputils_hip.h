@@ -1,0 +1,165 @@
+/*
+ * gputils_hip.h -- C ABI of libgputils_hip.so, the MI355X (gfx950) kernels
+ * behind GPUtils' batched dense linear-algebra API.
+ *
+ * Conventions
+ *  - Matrices are column-major with an explicit leading dimension.  A batch
+ *    of matrices is addressed as base pointer + element stride between
+ *    consecutive matrices (a GPUtils DTensor (m,n,k) has stride m*n).
+ *  - Sizes are int64_t.  `stream` is a hipStream_t passed as void*
+ *    (NULL = the null stream).  All compute calls are asynchronous on that
+ *    stream, allocate nothing and are therefore capturable in HIP graphs.
+ *  - Return value: 0 on success; -i if argument i (1-based) is invalid
+ *    (LAPACK convention, nothing is launched); a positive value for a HIP
+ *    runtime failure.  gputils_last_error() returns a message for the last
+ *    failing call of the calling thread.
+ *  - Numerical failures are reported per matrix through device `info`
+ *    arrays (LAPACK convention: 0 = success, i > 0 = failure at step i).
+ *
+ * Each entry point names the place in GPUtils' include/tensor.cuh whose
+ * vendor-library call it replaces.
+ */
+#ifndef GPUTILS_HIP_H
+#define GPUTILS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPUTILS_OK 0
+
+/* ------------------------------------------------------------ runtime --- */
+/* Message describing the last failing call on this thread ("" if none). */
+const char *gputils_last_error(void);
+/* Library version string. */
+const char *gputils_version(void);
+/* Number of visible GPUs (0 when no GPU is present). */
+int gputils_device_count(int *count);
+
+/* Device allocation; replaces cudaMalloc/cudaFree of Session::cudaAllocate
+ * and DTensor::destroy (tensor.cuh:209-215, 283-294). */
+int gputils_malloc(void **ptr, size_t bytes);
+int gputils_free(void *ptr);
+/* Synchronous copies with null-stream semantics (tensor.cuh:687, 893, 1142,
+ * 1150, 1204, 1282).  kind: 0 host->device, 1 device->host,
+ * 2 device->device, 3 inferred from the pointers. */
+int gputils_memcpy(void *dst, const void *src, size_t bytes, int kind);
+int gputils_memcpy_async(void *dst, const void *src, size_t bytes, int kind, void *stream);
+/* Synchronous memset (tensor.cuh:1114) and its stream-ordered variant. */
+int gputils_memset(void *ptr, int value, size_t bytes);
+/* Byte fill by a kernel (no memset node, so it is safe inside HIP graphs). */
+int gputils_fill_bytes_async(void *ptr, unsigned char value, size_t bytes, void *stream);
+
+/* Streams (Session, tensor.cuh:154-166, 232-246).  Streams are created
+ * blocking with default priority. */
+int gputils_stream_create(void **stream);
+int gputils_stream_destroy(void *stream);
+int gputils_stream_synchronize(void *stream);
+int gputils_device_synchronize(void);
+
+/* Device array of per-matrix pointers base + i*stride_bytes, i < count,
+ * built by a kernel (DTensor::initialisePointersToMatricesData,
+ * tensor.cuh:672-688, does this with a host loop and a copy). */
+int gputils_fill_pointer_array(void **dst, const void *base, int64_t stride_bytes,
+                               int64_t count, void *stream);
+
+/* --------------------------------------------------- BLAS-1 reductions --- */
+/* Whole-array reductions over n contiguous elements.  The result is written
+ * to *result (device memory).  `workspace` is device memory of at least
+ * gputils_reduce_workspace_bytes() bytes.
+ * Replace cublas?dot (tensor.cuh:968-992), ?nrm2 (994-1010), ?asum
+ * (1012-1028), I?amax/I?amin + one element copy (1030-1072: maxAbs/minAbs
+ * return |x| at that index, which is the max/min of |x|). */
+size_t gputils_reduce_workspace_bytes(void);
+int gputils_sdot(int64_t n, const float *x, const float *y, float *result, void *workspace, void *stream);
+int gputils_ddot(int64_t n, const double *x, const double *y, double *result, void *workspace, void *stream);
+int gputils_snrm2(int64_t n, const float *x, float *result, void *workspace, void *stream);
+int gputils_dnrm2(int64_t n, const double *x, double *result, void *workspace, void *stream);
+int gputils_sasum(int64_t n, const float *x, float *result, void *workspace, void *stream);
+int gputils_dasum(int64_t n, const double *x, double *result, void *workspace, void *stream);
+int gputils_samax(int64_t n, const float *x, float *result, void *workspace, void *stream);
+int gputils_damax(int64_t n, const double *x, double *result, void *workspace, void *stream);
+int gputils_samin(int64_t n, const float *x, float *result, void *workspace, void *stream);
+int gputils_damin(int64_t n, const double *x, double *result, void *workspace, void *stream);
+
+/* x <- alpha x (cublas?scal, tensor.cuh:1210-1237). */
+int gputils_sscal(int64_t n, float alpha, float *x, void *stream);
+int gputils_dscal(int64_t n, double alpha, double *x, void *stream);
+/* y <- y + alpha x (cublas?axpy, tensor.cuh:1239-1276). */
+int gputils_saxpy(int64_t n, float alpha, const float *x, float *y, void *stream);
+int gputils_daxpy(int64_t n, double alpha, const double *x, double *y, void *stream);
+
+/* B_i = A_i^T, A_i m x n, B_i n x m (cublas?geam per matrix,
+ * tensor.cuh:1167-1197). */
+int gputils_stranspose_batched(int64_t m, int64_t n, const float *A, int64_t lda, int64_t strideA,
+                               float *B, int64_t ldb, int64_t strideB, int64_t batch, void *stream);
+int gputils_dtranspose_batched(int64_t m, int64_t n, const double *A, int64_t lda, int64_t strideA,
+                               double *B, int64_t ldb, int64_t strideB, int64_t batch, void *stream);
+
+/* ---------------------------------------------------------------- GEMM --- */
+/* C_i <- alpha A_i B_i + beta C_i; A_i m x k, B_i k x n, C_i m x n
+ * (cublas?gemmBatched / ?gemm, tensor.cuh:1286-1338).  C may alias B
+ * exactly (same pointer, ld and stride), as Nullspace::project does
+ * (tensor.cuh:2084): every matrix is then read completely before it is
+ * overwritten.  Other partial overlaps are not supported.
+ * mode (float only): GPUTILS_GEMM_FP32 = fp32 products and fp32
+ * accumulation on MFMA/VALU; GPUTILS_GEMM_BF16X3 = each operand split into
+ * two bfloat16 terms, three bf16 MFMA products with fp32 accumulation. */
+#define GPUTILS_GEMM_FP32 0
+#define GPUTILS_GEMM_BF16X3 1
+int gputils_sgemm_strided_batched(int64_t m, int64_t n, int64_t k, float alpha,
+                                  const float *A, int64_t lda, int64_t strideA,
+                                  const float *B, int64_t ldb, int64_t strideB, float beta,
+                                  float *C, int64_t ldc, int64_t strideC, int64_t batch,
+                                  int mode, void *stream);
+int gputils_dgemm_strided_batched(int64_t m, int64_t n, int64_t k, double alpha,
+                                  const double *A, int64_t lda, int64_t strideA,
+                                  const double *B, int64_t ldb, int64_t strideB, double beta,
+                                  double *C, int64_t ldc, int64_t strideC, int64_t batch,
+                                  void *stream);
+
+/* ------------------------------------------------------------ Cholesky --- */
+/* Lower Cholesky A_i = L_i L_i^T in place; the strictly upper triangle is
+ * not referenced.  info[i] = 0 or the order of the first non-positive
+ * leading minor (cusolverDn?potrfBatched, tensor.cuh:2136-2159). */
+int gputils_spotrf_batched(int64_t n, float *A, int64_t lda, int64_t strideA, int *info,
+                           int64_t batch, void *stream);
+int gputils_dpotrf_batched(int64_t n, double *A, int64_t lda, int64_t strideA, int *info,
+                           int64_t batch, void *stream);
+/* Solve L_i L_i^T X_i = B_i in place for nrhs >= 1 columns
+ * (cusolverDn?potrsBatched, tensor.cuh:2161-2197, which only allows
+ * nrhs = 1).  info is written with 0 (argument checks happen on the host). */
+int gputils_spotrs_batched(int64_t n, int64_t nrhs, const float *L, int64_t lda, int64_t strideL,
+                           float *B, int64_t ldb, int64_t strideB, int *info, int64_t batch,
+                           void *stream);
+int gputils_dpotrs_batched(int64_t n, int64_t nrhs, const double *L, int64_t lda, int64_t strideL,
+                           double *B, int64_t ldb, int64_t strideB, int *info, int64_t batch,
+                           void *stream);
+/* Fused factor + solve in FP64: A_i <- L_i, B_i <- A_i^{-1} B_i in one
+ * launch (the pair factorise(); solve(b) of CholeskyBatchFactoriser). */
+int gputils_dposv_batched(int64_t n, int64_t nrhs, double *A, int64_t lda, int64_t strideA,
+                          double *B, int64_t ldb, int64_t strideB, int *info, int64_t batch,
+                          void *stream);
+/* Mixed-precision solve (no reference counterpart: a new entry point next
+ * to CholeskyBatchFactoriser).  Factor fl32(A_i) on the GPU, solve in
+ * FP32 and refine in FP64 until, for every right-hand side,
+ *   ||b - A x||_inf <= tol * (||A||_inf ||x||_inf + ||b||_inf),
+ * or max_iter refinement steps were made.  A_i and B_i are read only
+ * (FP64, lower triangle of A_i used); X_i receives the FP64 solution.
+ * iters[i] = refinement steps used, or -1 if the FP32 factorisation failed
+ * or refinement did not converge; those matrices are then solved by a
+ * second, FP64 launch (as LAPACK DSPOSV falls back), which also reads A_i
+ * and B_i only, and info[i] holds the FP64 status (0 or the failing minor).
+ * Requires n <= 128. */
+int gputils_dposv_mixed_batched(int64_t n, int64_t nrhs, const double *A, int64_t lda,
+                                int64_t strideA, const double *B, int64_t ldb, int64_t strideB,
+                                double *X, int64_t ldx, int64_t strideX, double tol, int max_iter,
+                                int *iters, int *info, int64_t batch, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPUTILS_HIP_H */

@@ -1,0 +1,241 @@
+"""GPU parity of the batched Cholesky kernels against the FP64 oracle and
+the reference suite's golden values (test/testTensor.cu X:1218-1413).
+
+Tolerances: FP64 kernels agree with the oracle to 1e-12 (relative to the
+matrix scale); FP32 kernels to 1e-4 (the reference suite's PRECISION_LOW);
+the mixed solver must reach the requested backward error 1e-12."""
+import numpy as np
+import pytest
+import torch
+
+import golden_values as G
+from gputils_amd import api
+
+pytestmark = pytest.mark.gpu
+
+
+def spd_batch(k, n, seed, shift=None):
+    r = np.random.default_rng(seed)
+    M = r.standard_normal((k, n, n))
+    return M @ np.swapaxes(M, 1, 2) + (n if shift is None else shift) * np.eye(n)
+
+
+def dev(a, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.cuda()
+
+
+def backward_error(A, X, B):
+    """max over matrices/columns of ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf).
+    A (k,n,n) matrices, X and B in tensor layout (k, nrhs, n)."""
+    R = B - np.einsum("kij,krj->kri", A, X)
+    an = np.abs(A).sum(axis=2).max(axis=1)[:, None]
+    return float(np.max(np.abs(R).max(axis=2) / (an * np.abs(X).max(axis=2) + np.abs(B).max(axis=2))))
+
+
+# ------------------------------------------------------------- goldens --
+@pytest.mark.parametrize("dtype,eps", [(torch.float32, G.PRECISION_LOW), (torch.float64, G.PRECISION_HIGH)])
+def test_batch_factorisation_golden(dtype, eps):
+    A = np.stack([G.cm(G.CHOL_A, 3, 3)[0]] * 2)
+    t = dev(A, dtype)
+    info = api.potrf_batched(t)
+    torch.cuda.synchronize()
+    L = api.to_matrices(t).cpu().double().numpy()
+    for b in range(2):
+        assert abs(L[b, 0, 0] - G.CHOL_L00) < eps
+        assert abs(L[b, 2, 1] - G.CHOL_L21) < eps
+        assert abs(L[b, 2, 2] - G.CHOL_L22) < eps
+    assert info.cpu().tolist() == [0, 0]
+
+
+@pytest.mark.parametrize("dtype,eps", [(torch.float32, G.PRECISION_LOW), (torch.float64, G.PRECISION_HIGH)])
+def test_batch_factor_solve_golden(dtype, eps):
+    A = np.stack([G.cm(G.CHOL_A, 3, 3)[0]] * 2)
+    L = dev(A, dtype)
+    api.potrf_batched(L)
+    b = dev(np.tile(np.array(G.CHOL_B), (2, 1, 1)), dtype)
+    api.potrs_batched(L, b)
+    x = b.cpu().double().numpy()
+    for k in range(2):
+        assert np.allclose(x[k, 0], G.CHOL_X, atol=eps)
+        assert np.linalg.norm(A[k] @ x[k, 0] - np.array(G.CHOL_B)) < eps
+
+
+@pytest.mark.parametrize("dtype,eps", [(torch.float32, G.PRECISION_LOW), (torch.float64, G.PRECISION_HIGH)])
+def test_batch_solve_from_given_factor_golden(dtype, eps):
+    Lg = np.array(G.CHOL_L_ROWMAJOR).reshape(3, 3)
+    L = dev(api.from_matrices(torch.from_numpy(np.stack([Lg, Lg]))).numpy(), dtype)
+    b = dev(np.tile(np.array(G.CHOL_B), (2, 1, 1)), dtype)
+    api.potrs_batched(L, b)
+    x = b.cpu().double().numpy()
+    for k in range(2):
+        assert np.allclose(x[k, 0], G.CHOL_X, atol=eps)
+
+
+# ------------------------------------------------------- oracle parity --
+@pytest.mark.parametrize("n", [1, 3, 15, 16, 17, 31, 32, 33, 64, 77, 100, 128, 129, 150])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_potrf_matches_oracle(oracle, n, dtype):
+    k = 5
+    A = spd_batch(k, n, seed=n)
+    A_in = A.copy()
+    iu = np.triu_indices(n, 1)
+    for b in range(k):
+        A_in[b][iu] = 777.0  # strictly upper part: must be ignored and left untouched
+    t = dev(api.from_matrices(torch.from_numpy(A_in)).numpy(), dtype)
+    info = api.potrf_batched(t)
+    torch.cuda.synchronize()
+    got = api.to_matrices(t).cpu().double().numpy()
+    assert info.cpu().tolist() == [0] * k
+    tol = 1e-12 if dtype == torch.float64 else 2e-4
+    for b in range(k):
+        L, oinfo = oracle.potrf(A[b])
+        assert oinfo == 0
+        scale = np.abs(np.tril(L)).max()
+        assert np.max(np.abs(np.tril(got[b]) - np.tril(L))) <= tol * scale, (n, b)
+        assert np.all(got[b][iu] == 777.0)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_potrf_reports_failing_minor(oracle, dtype):
+    n, k = 40, 6
+    A = spd_batch(k, n, seed=7)
+    A[1][5, 5] = -3.0            # fails at row 6
+    A[3][:, :] = 0.0             # fails at row 1
+    A[4][39, 39] = -1e6          # fails at the last row
+    A[5][20, 20] = np.nan        # fails at row 21
+    t = dev(A, dtype)
+    info = api.potrf_batched(t).cpu().tolist()
+    expected = [oracle.potrf(A[b])[1] for b in range(k)]
+    assert expected == [0, 6, 0, 1, 40, 21]
+    assert info == expected
+
+
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (3, 1), (16, 4), (32, 1), (45, 3), (64, 5), (128, 4), (128, 9), (140, 2)])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_potrs_matches_oracle(oracle, n, nrhs, dtype):
+    k = 4
+    A = spd_batch(k, n, seed=100 + n)
+    B = np.random.default_rng(n).standard_normal((k, nrhs, n))
+    L = dev(A, dtype)
+    api.potrf_batched(L)
+    X = dev(B, dtype)
+    api.potrs_batched(L, X)
+    got = X.cpu().double().numpy()
+    tol = 1e-11 if dtype == torch.float64 else 1e-4
+    for b in range(k):
+        Lo, _ = oracle.potrf(A[b])
+        ref = oracle.potrs(Lo, B[b].T)
+        assert np.max(np.abs(got[b].T - ref)) <= tol * max(1.0, np.abs(ref).max()), (n, nrhs, b)
+
+
+@pytest.mark.parametrize("n,nrhs", [(3, 1), (32, 1), (50, 4), (128, 4), (128, 6)])
+def test_fused_posv_matches_oracle(oracle, n, nrhs):
+    k = 6
+    A = spd_batch(k, n, seed=200 + n)
+    B = np.random.default_rng(n).standard_normal((k, nrhs, n))
+    tA, tB = dev(A), dev(B)
+    info = api.posv_batched(tA, tB)
+    L = api.to_matrices(tA).cpu().numpy()
+    X = tB.cpu().numpy()
+    assert info.cpu().tolist() == [0] * k
+    for b in range(k):
+        Lo, _ = oracle.potrf(A[b])
+        assert np.max(np.abs(np.tril(L[b]) - np.tril(Lo))) <= 1e-12 * np.abs(Lo).max()
+        ref = oracle.potrs(Lo, B[b].T)
+        assert np.max(np.abs(X[b].T - ref)) <= 1e-11 * np.abs(ref).max()
+
+
+# ---------------------------------------------------- mixed precision --
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (8, 1), (16, 2), (33, 4), (100, 3), (128, 4), (128, 7)])
+def test_mixed_reaches_fp64_accuracy(oracle, n, nrhs):
+    k = 16
+    A = spd_batch(k, n, seed=300 + n, shift=0.5)
+    B = np.random.default_rng(n + 1).standard_normal((k, nrhs, n))
+    X, iters, info = api.posv_mixed_batched(dev(A), dev(B), tol=1e-12)
+    X = X.cpu().numpy()
+    assert info.cpu().tolist() == [0] * k
+    it = iters.cpu().numpy()
+    assert np.all(it >= 0) and np.all(it <= 30)
+    assert backward_error(A, X, B) <= 1e-12
+    for b in range(k):
+        Lo, _ = oracle.potrf(A[b])
+        ref = oracle.potrs(Lo, B[b].T)
+        cond = np.linalg.cond(A[b])
+        assert np.max(np.abs(X[b].T - ref)) <= 1e-12 * cond * np.abs(ref).max()
+
+
+def test_mixed_falls_back_to_fp64(oracle):
+    """cond ~ 1e12 (FP32 factor unusable) and a non-SPD matrix: the FP64
+    launch takes over; iters = -1 marks them."""
+    n, k = 48, 4
+    r = np.random.default_rng(5)
+    Q, _ = np.linalg.qr(r.standard_normal((n, n)))
+    A = spd_batch(k, n, seed=9)
+    A[1] = (Q * np.logspace(0, -12, n)) @ Q.T
+    A[1] = 0.5 * (A[1] + A[1].T)
+    A[2] = -A[2]
+    B = r.standard_normal((k, 1, n))
+    X, iters, info = api.posv_mixed_batched(dev(A), dev(B), tol=1e-12, max_iter=10)
+    it, inf = iters.cpu().tolist(), info.cpu().tolist()
+    X = X.cpu().numpy()
+    assert it[0] >= 0 and it[3] >= 0
+    assert it[1] == -1 and it[2] == -1
+    assert inf[0] == 0 and inf[2] == 1 and inf[3] == 0
+    Lo, oinfo = oracle.potrf(A[1])
+    assert inf[1] == oinfo
+    if oinfo == 0:
+        ref = oracle.potrs(Lo, B[1].T)
+        assert np.allclose(X[1].T, ref, rtol=1e-2, atol=1e-2 * np.abs(ref).max())
+    for b in (0, 3):
+        assert backward_error(A[b:b + 1], X[b:b + 1], B[b:b + 1]) <= 1e-12
+
+
+def test_mixed_inputs_are_read_only():
+    n, k = 64, 3
+    A = spd_batch(k, n, seed=11)
+    B = np.random.default_rng(2).standard_normal((k, 2, n))
+    tA, tB = dev(A), dev(B)
+    api.posv_mixed_batched(tA, tB)
+    assert np.array_equal(tA.cpu().numpy(), A) and np.array_equal(tB.cpu().numpy(), B)
+
+
+# --------------------------------------------------- benchmark configs --
+def test_config_cholesky_small_fp64(oracle):
+    """k=1024 SPD 32x32 (M M^T + 32 I), b 32x1: factor + solve in FP64,
+    relative residual <= 1e-13, and agreement with the oracle."""
+    from gputils_amd import workloads
+    A, b = workloads.cholesky_small()
+    tA, tb = dev(A), dev(b)
+    info = api.potrf_batched(tA)
+    api.potrs_batched(tA, tb)
+    x = tb.cpu().numpy()
+    assert int(info.abs().sum()) == 0
+    res = np.linalg.norm(b - np.einsum("kij,krj->kri", A, x), axis=2).ravel()
+    rel = res / np.linalg.norm(b, axis=2).ravel()
+    assert rel.max() <= 1e-13
+    A2, b2 = A[:64].copy(), b[:64].copy()
+    assert oracle.posv_batched(A2, b2) == 0
+    assert np.max(np.abs(x[:64] - b2)) <= 1e-13 * np.abs(b2).max()
+
+
+def test_config_cholesky_mixed_full_size():
+    """k=8192 SPD 128x128 (lambda in [1,1e4]), 4 RHS: every system reaches
+    backward error <= 1e-12 with the mixed solver; the plain FP64 solve of the
+    same systems agrees to cond * 1e-12."""
+    from gputils_amd import workloads
+    A, B = workloads.cholesky_mixed()
+    tA, tB = dev(A), dev(B)
+    X, iters, info = api.posv_mixed_batched(tA, tB, tol=1e-12)
+    Xm = X.cpu().numpy()
+    it = iters.cpu().numpy()
+    assert int(info.abs().sum()) == 0
+    assert np.all(it >= 0), "FP64 fallback was needed"
+    assert backward_error(A, Xm, B) <= 1e-12
+    tB2 = tB.clone()
+    api.posv_batched(tA, tB2)   # overwrites tA with L
+    Xp = tB2.cpu().numpy()
+    assert backward_error(A, Xp, B) <= 1e-12
+    assert np.max(np.abs(Xm - Xp)) <= 1e4 * 1e-12 * np.abs(Xp).max()
