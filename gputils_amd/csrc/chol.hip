@@ -31,60 +31,90 @@ constexpr int kRhsChunk = 4;  // right-hand sides per solve sweep (lane = row + 
 __host__ __device__ constexpr int tri(int I) { return (I * (I + 1)) >> 1; }
 
 // Load the lower triangle of the n x n matrix A (column-major, lda) into
-// LDS block storage with identity padding up to 16*nb.  Columns are read
-// whole (contiguous rows) by one wavefront each.
+// LDS block storage with identity padding up to 16*nb.  Thread t owns
+// element (t % 16, t / 16) of every block; blocks are fetched eight at a
+// time so that each thread keeps eight independent loads in flight.
 template <typename Tin, typename T>
-__device__ void load_lower(const Tin *__restrict__ A, int64_t lda, int n, int nb, T *L) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int npad = nb * 16;
-    for (int gk = wave; gk < npad; gk += kWaves) {
-        const int J = gk >> 4, k = gk & 15;
-        for (int gi = (gk & ~15) + lane; gi < npad; gi += kWave) {
-            T v;
-            if (gi < n && gk < n)
-                v = (gi >= gk) ? (T)A[gi + (int64_t)gk * lda] : T(0);
-            else
-                v = (gi == gk) ? T(1) : T(0);
-            const int I = gi >> 4;
-            L[(tri(I) + J) * 256 + (gi & 15) + 16 * k] = v;
+__device__ __forceinline__ void load_lower(const Tin *__restrict__ A, int64_t lda, int n, int nb, T *L) {
+    const int i = threadIdx.x & 15, k = threadIdx.x >> 4;
+    const int nblk = tri(nb);
+    int I = 0, J = 0;  // coordinates of block b0 (uniform)
+    for (int b0 = 0; b0 < nblk; b0 += 8) {
+        T v[8];
+        int Iu = I, Ju = J;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int gi = 16 * Iu + i, gk = 16 * Ju + k;
+            T x = (gi == gk) ? T(1) : T(0);
+            if (b0 + u < nblk && gi < n && gk < n && gi >= gk) x = (T)A[gi + (int64_t)gk * lda];
+            v[u] = x;
+            if (++Ju > Iu) { ++Iu; Ju = 0; }
         }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (b0 + u < nblk) L[(b0 + u) * 256 + i + 16 * k] = v[u];
+        I = Iu;
+        J = Ju;
     }
 }
 
 // Store the strictly-below-diagonal blocks (I > J) back to A.
 template <typename T>
-__device__ void store_offdiag(T *__restrict__ A, int64_t lda, int n, int nb, const T *L) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int gk = wave; gk < n; gk += kWaves) {
-        const int J = gk >> 4, k = gk & 15;
-        for (int gi = 16 * (J + 1) + lane; gi < n; gi += kWave) {
-            const int I = gi >> 4;
-            A[gi + (int64_t)gk * lda] = L[(tri(I) + J) * 256 + (gi & 15) + 16 * k];
+__device__ __forceinline__ void store_offdiag(T *__restrict__ A, int64_t lda, int n, int nb, const T *L) {
+    const int i = threadIdx.x & 15, k = threadIdx.x >> 4;
+    for (int I = 1; I < nb; ++I) {
+        const int gi = 16 * I + i;
+        for (int J = 0; J < I; ++J) {
+            const int gk = 16 * J + k;
+            if (gi < n && gk < n) A[gi + (int64_t)gk * lda] = L[(tri(I) + J) * 256 + i + 16 * k];
         }
     }
 }
 
-// Factor the 16x16 diagonal block `blk` (lower part valid) with wave 0,
-// write D = L^{-1} over it.  If Lout != nullptr the factor L itself is
-// written there (lower part, rows < nvalid, column-major with ldo).
+// Pivot of column k: s = sqrt(d) and inv = 1/s.  FP32 uses the hardware
+// reciprocal square root (v_rsq_f32, 1 ulp); FP64 keeps IEEE sqrt/divide.
+__device__ __forceinline__ void pivot(float d, float &s, float &inv) {
+    inv = __builtin_amdgcn_rsqf(d);
+    s = d * inv;
+}
+__device__ __forceinline__ void pivot(double d, double &s, double &inv) {
+    s = __builtin_sqrt(d);
+    inv = 1.0 / s;
+}
+
+// In registers of one wavefront (lane r < 16 holds row r of the block),
+// invert the lower triangular L (row[]) given the reciprocals of its
+// diagonal (uniform): lane c returns column c of L^{-1} in x[].
+template <typename T>
+__device__ __forceinline__ void tri_inverse_cols(const T (&row)[16], const T (&invd)[16], T (&x)[16]) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        T acc = (lane == r) ? T(1) : T(0);
+#pragma unroll
+        for (int p = 0; p < r; ++p) acc -= readlane(row[p], r) * x[p];
+        x[r] = acc * invd[r];
+    }
+}
+
+// Factor the 16x16 diagonal block `blk` (lower part valid) with one
+// wavefront and write D = L^{-1} over it (column-major).  If Lout != nullptr
+// the factor L itself is written there (rows/cols < nvalid).
 // Returns the index (0..15) of the first non-positive pivot or -1.
 template <typename T>
-__device__ int factor_diag_block(T *blk, T *Lout, int64_t ldo, int nvalid) {
+__device__ __forceinline__ int factor_diag_block(T *blk, T *Lout, int64_t ldo, int nvalid) {
     const int lane = threadIdx.x & 63;
-    T row[16];
+    T row[16], invd[16];
 #pragma unroll
-    for (int c = 0; c < 16; ++c) row[c] = (lane < 16) ? blk[lane + 16 * c] : T(0);
-    int fail = -1;
+    for (int c = 0; c < 16; ++c) row[c] = blk[(lane & 15) + 16 * c];
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
         const T d = readlane(row[k], k);
-        if (!(d > T(0))) {  // uniform: also catches NaN
-            fail = k;
-            break;
-        }
-        const T s = dev_sqrt(d);
-        const T inv = T(1) / s;
-        const T lk = (lane > k) ? row[k] * inv : ((lane == k) ? s : T(0));
+        if (!(d > T(0))) return k;  // uniform; also catches NaN
+        T sq, inv;
+        pivot(d, sq, inv);
+        invd[k] = inv;
+        const T lk = (lane > k) ? row[k] * inv : ((lane == k) ? sq : T(0));
         row[k] = lk;
 #pragma unroll
         for (int j = k + 1; j < 16; ++j) {
@@ -92,22 +122,13 @@ __device__ int factor_diag_block(T *blk, T *Lout, int64_t ldo, int nvalid) {
             if (lane >= j) row[j] -= lk * ljk;
         }
     }
-    if (fail >= 0) return fail;
     if (Lout != nullptr && lane < nvalid) {
 #pragma unroll
         for (int c = 0; c < 16; ++c)
             if (c <= lane && c < nvalid) Lout[lane + (int64_t)c * ldo] = row[c];
     }
-    // D = L^{-1}: lane c builds column c by forward substitution; the
-    // L entries are uniform (v_readlane of row r).
     T x[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        T acc = (lane == r) ? T(1) : T(0);
-#pragma unroll
-        for (int p = 0; p < r; ++p) acc -= readlane(row[p], r) * x[p];
-        x[r] = acc / readlane(row[r], r);
-    }
+    tri_inverse_cols(row, invd, x);
     if (lane < 16) {
 #pragma unroll
         for (int r = 0; r < 16; ++r) blk[r + 16 * lane] = x[r];
@@ -115,76 +136,113 @@ __device__ int factor_diag_block(T *blk, T *Lout, int64_t ldo, int nvalid) {
     return -1;
 }
 
-// Blocked right-looking factorisation of the LDS block triangle.
-// On return the diagonal slots hold D_J = L_JJ^{-1}; off-diagonal slots
-// hold L_IJ.  When Lout != nullptr each L_JJ is written to Lout as it is
-// formed.  Returns 0 or the 1-based failing row (LAPACK info).
+// C -= A_op * B_op^T for one 16x16 tile (all operands 16x16 LDS blocks,
+// column-major); two tiles are interleaved for MFMA latency hiding.
 template <typename T>
-__device__ int factor_blocked(T *L, int nb, int n, T *Lout, int64_t ldo, int *s_flag) {
+__device__ __forceinline__ void tile_update2(T *c0, const T *a0, const T *b0, T *c1, const T *a1,
+                                    const T *b1, bool two) {
+    using M = Mfma16<T>;
+    const int lane = threadIdx.x & 63;
+    const int col = 16 * (lane & 15);
+    typename M::acc_t x0, x1;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) x0[r] = c0[M::drow(lane, r) + col];
+    if (two) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) x1[r] = c1[M::drow(lane, r) + col];
+    }
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+        x0 = M::fma(-a0[lane + 64 * kk], b0[lane + 64 * kk], x0);
+        if (two) x1 = M::fma(-a1[lane + 64 * kk], b1[lane + 64 * kk], x1);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) c0[M::drow(lane, r) + col] = x0[r];
+    if (two) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) c1[M::drow(lane, r) + col] = x1[r];
+    }
+}
+
+// Blocked right-looking factorisation of the LDS block triangle with
+// lookahead: while waves 1..3 apply the trailing update of block column
+// J, wave 0 first updates the next diagonal block and factors it, so the
+// serial 16x16 factorisations overlap the MFMA work.  Two barriers per
+// block column.  On return the diagonal slots hold D_J = L_JJ^{-1} and the
+// off-diagonal slots L_IJ; each L_JJ goes to Lout when Lout != nullptr.
+// Returns 0 or the 1-based failing row (LAPACK info).
+template <typename T>
+__device__ __forceinline__ int factor_blocked(T *L, int nb, int n, T *Lout, int64_t ldo, int *s_flag) {
     using M = Mfma16<T>;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    auto blk = [&](int I, int J) { return L + (tri(I) + J) * 256; };
+    auto lout = [&](int J) { return (Lout != nullptr) ? Lout + 16 * J + (int64_t)16 * J * ldo : (T *)nullptr; };
+    if (wave == 0) {
+        const int f = factor_diag_block(blk(0, 0), lout(0), ldo, n);
+        if (lane == 0) *s_flag = (f >= 0) ? f + 1 : 0;
+    }
+    __syncthreads();
+    if (*s_flag) return *s_flag;
     for (int J = 0; J < nb; ++J) {
-        T *dJ = L + (tri(J) + J) * 256;
-        if (wave == 0) {
-            T *lo = (Lout != nullptr) ? Lout + 16 * J + (int64_t)16 * J * ldo : nullptr;
-            const int f = factor_diag_block(dJ, lo, ldo, n - 16 * J);
-            if (lane == 0) *s_flag = (f >= 0) ? 16 * J + f + 1 : 0;
-        }
-        __syncthreads();
-        const int fail = *s_flag;
-        if (fail) return fail;
+        const T *dJ = blk(J, J);
         // panel: L_IJ = A_IJ * D_J^T
         for (int I = J + 1 + wave; I < nb; I += kWaves) {
-            T *t = L + (tri(I) + J) * 256;
+            T *t = blk(I, J);
             typename M::acc_t acc = M::zero();
 #pragma unroll
             for (int kk = 0; kk < 4; ++kk) acc = M::fma(t[lane + 64 * kk], dJ[lane + 64 * kk], acc);
-            const int base = 4 * (lane >> 4) + 16 * (lane & 15);
+            const int col = 16 * (lane & 15);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) t[base + r] = acc[r];
+            for (int r = 0; r < 4; ++r) t[M::drow(lane, r) + col] = acc[r];
         }
         __syncthreads();
-        // trailing update of tiles (I,K), J < K <= I
-        int t = 0;
-        for (int K = J + 1; K < nb; ++K) {
-            const T *lk = L + (tri(K) + J) * 256;
-            for (int I = K; I < nb; ++I, ++t) {
-                if ((t & (kWaves - 1)) != wave) continue;
-                const T *li = L + (tri(I) + J) * 256;
-                T *c = L + (tri(I) + K) * 256;
-                const int base = 4 * (lane >> 4) + 16 * (lane & 15);
-                typename M::acc_t acc;
-#pragma unroll
-                for (int r = 0; r < 4; ++r) acc[r] = c[base + r];
-#pragma unroll
-                for (int kk = 0; kk < 4; ++kk) acc = M::fma(-li[lane + 64 * kk], lk[lane + 64 * kk], acc);
-#pragma unroll
-                for (int r = 0; r < 4; ++r) c[base + r] = acc[r];
+        if (J + 1 == nb) break;
+        if (wave == 0) {
+            // lookahead: next diagonal tile first, then its factorisation
+            tile_update2(blk(J + 1, J + 1), blk(J + 1, J), blk(J + 1, J), (T *)nullptr, (const T *)nullptr,
+                         (const T *)nullptr, false);
+            const int f = factor_diag_block(blk(J + 1, J + 1), lout(J + 1), ldo, n - 16 * (J + 1));
+            if (lane == 0) *s_flag = (f >= 0) ? 16 * (J + 1) + f + 1 : 0;
+        } else {
+            // remaining tiles (I,K), J < K <= I, except (J+1,J+1), over waves 1..3
+            int t = 0, pend = -1;
+            T *pc = nullptr;
+            const T *pa = nullptr, *pb = nullptr;
+            for (int K = J + 1; K < nb; ++K) {
+                for (int I = K; I < nb; ++I) {
+                    if (I == J + 1 && K == J + 1) continue;
+                    if ((t++ % (kWaves - 1)) != wave - 1) continue;
+                    if (pend < 0) {
+                        pc = blk(I, K);
+                        pa = blk(I, J);
+                        pb = blk(K, J);
+                        pend = 1;
+                    } else {
+                        tile_update2(pc, pa, pb, blk(I, K), blk(I, J), blk(K, J), true);
+                        pend = -1;
+                    }
+                }
             }
+            if (pend > 0) tile_update2(pc, pa, pb, (T *)nullptr, (const T *)nullptr, (const T *)nullptr, false);
         }
         __syncthreads();
+        if (*s_flag) return *s_flag;
     }
     return 0;
 }
 
 // In LDS, replace each diagonal block L_JJ by its inverse (wave J % 4).
 template <typename T>
-__device__ void invert_diag_blocks(T *L, int nb) {
-    const int wave = threadIdx.x >> 6;
+__device__ __forceinline__ void invert_diag_blocks(T *L, int nb) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int J = wave; J < nb; J += kWaves) {
         T *dJ = L + (tri(J) + J) * 256;
-        const int lane = threadIdx.x & 63;
-        T row[16];
+        T row[16], invd[16], x[16];
 #pragma unroll
-        for (int c = 0; c < 16; ++c) row[c] = (lane < 16) ? dJ[lane + 16 * c] : T(0);
-        T x[16];
+        for (int c = 0; c < 16; ++c) row[c] = dJ[(lane & 15) + 16 * c];
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            T acc = (lane == r) ? T(1) : T(0);
-#pragma unroll
-            for (int p = 0; p < r; ++p) acc -= readlane(row[p], r) * x[p];
-            x[r] = acc / readlane(row[r], r);
-        }
+        for (int r = 0; r < 16; ++r) invd[r] = T(1) / readlane(row[r], r);
+        tri_inverse_cols(row, invd, x);
         if (lane < 16) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) dJ[r + 16 * lane] = x[r];
@@ -192,55 +250,77 @@ __device__ void invert_diag_blocks(T *L, int nb) {
     }
 }
 
-// Solve L L^T Y = Y in place for the npad x 4 block Y (LDS, column-major,
-// ld = npad) using the block triangle (diagonal slots = inverses).
-// Lane = row + 16 * rhs inside a 16-row block.
+// y_i = sum_k M[i + 16k] * v[k] (M = D or L block), two partial sums.
 template <typename T>
-__device__ void solve_blocked(const T *L, int nb, T *Y) {
+__device__ __forceinline__ T dot16(const T *m, int i, int stride_i, int stride_k, const T *v) {
+    T a0 = T(0), a1 = T(0);
+#pragma unroll
+    for (int k = 0; k < 16; k += 2) {
+        a0 += m[i * stride_i + k * stride_k] * v[k];
+        a1 += m[i * stride_i + (k + 1) * stride_k] * v[k + 1];
+    }
+    return a0 + a1;
+}
+
+// Solve L L^T Y = Y in place for the npad x 4 block Y (LDS, column-major,
+// ld = npad) with the block triangle (diagonal slots = inverses).  Lane =
+// row + 16 * rhs inside a 16-row block.  Lookahead: the wave that owns the
+// next block row applies the current update to it first and then its
+// diagonal product, so each block step costs one barrier.
+template <typename T>
+__device__ __forceinline__ void solve_blocked(const T *L, int nb, T *Y) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int i = lane & 15, r = lane >> 4;
     const int ld = 16 * nb;
+    auto blk = [&](int I, int J) { return L + (tri(I) + J) * 256; };
     // forward: L y = b
-    for (int I = 0; I < nb; ++I) {
-        if (wave == (I & (kWaves - 1))) {
-            const T *d = L + (tri(I) + I) * 256;
-            T *y = Y + 16 * I + ld * r;
-            T acc = T(0);
-#pragma unroll
-            for (int k = 0; k < 16; ++k) acc += d[i + 16 * k] * y[k];
-            y[i] = acc;
-        }
-        __syncthreads();
-        for (int K = I + 1 + wave; K < nb; K += kWaves) {
-            const T *l = L + (tri(K) + I) * 256;
-            const T *y = Y + 16 * I + ld * r;
-            T *x = Y + 16 * K + ld * r;
-            T acc = x[i];
-#pragma unroll
-            for (int k = 0; k < 16; ++k) acc -= l[i + 16 * k] * y[k];
-            x[i] = acc;
+    if (wave == 0) {
+        T *y = Y + ld * r;
+        const T v = dot16(blk(0, 0), i, 1, 16, y);
+        y[i] = v;
+    }
+    __syncthreads();
+    for (int I = 0; I + 1 < nb; ++I) {
+        const T *yI = Y + 16 * I + ld * r;
+        const int owner = (I + 1) & (kWaves - 1);
+        if (wave == owner) {
+            T *x = Y + 16 * (I + 1) + ld * r;
+            x[i] -= dot16(blk(I + 1, I), i, 1, 16, yI);
+            const T v = dot16(blk(I + 1, I + 1), i, 1, 16, x);
+            x[i] = v;
+        } else {
+            int t = 0;
+            for (int K = I + 2; K < nb; ++K, ++t) {
+                if ((t % (kWaves - 1)) != ((wave - owner - 1) & (kWaves - 1))) continue;
+                T *x = Y + 16 * K + ld * r;
+                x[i] -= dot16(blk(K, I), i, 1, 16, yI);
+            }
         }
         __syncthreads();
     }
     // backward: L^T x = y
-    for (int I = nb - 1; I >= 0; --I) {
-        if (wave == (I & (kWaves - 1))) {
-            const T *d = L + (tri(I) + I) * 256;
-            T *y = Y + 16 * I + ld * r;
-            T acc = T(0);
-#pragma unroll
-            for (int k = 0; k < 16; ++k) acc += d[k + 16 * i] * y[k];
-            y[i] = acc;
-        }
-        __syncthreads();
-        for (int K = wave; K < I; K += kWaves) {
-            const T *l = L + (tri(I) + K) * 256;
-            const T *z = Y + 16 * I + ld * r;
-            T *x = Y + 16 * K + ld * r;
-            T acc = x[i];
-#pragma unroll
-            for (int k = 0; k < 16; ++k) acc -= l[k + 16 * i] * z[k];
-            x[i] = acc;
+    const int last = nb - 1;
+    if (wave == (last & (kWaves - 1))) {
+        T *y = Y + 16 * last + ld * r;
+        const T v = dot16(blk(last, last), i, 16, 1, y);
+        y[i] = v;
+    }
+    __syncthreads();
+    for (int I = last; I > 0; --I) {
+        const T *zI = Y + 16 * I + ld * r;
+        const int owner = (I - 1) & (kWaves - 1);
+        if (wave == owner) {
+            T *x = Y + 16 * (I - 1) + ld * r;
+            x[i] -= dot16(blk(I, I - 1), i, 16, 1, zI);
+            const T v = dot16(blk(I - 1, I - 1), i, 16, 1, x);
+            x[i] = v;
+        } else {
+            int t = 0;
+            for (int K = I - 2; K >= 0; --K, ++t) {
+                if ((t % (kWaves - 1)) != ((wave - owner - 1) & (kWaves - 1))) continue;
+                T *x = Y + 16 * K + ld * r;
+                x[i] -= dot16(blk(I, K), i, 16, 1, zI);
+            }
         }
         __syncthreads();
     }
@@ -407,16 +487,114 @@ __global__ __launch_bounds__(kThreads) void potrs_global_kernel(int n, int nrhs,
 }
 
 // --------------------------------------------------- mixed precision --
-// FP32 factor in LDS; FP64 iterate x (LDS); FP64 residual straight from
-// the FP64 matrix in HBM/L2 (symmetric: element (p,q) = A[max + min*lda]).
+constexpr int kTileLd = 17;  // padded leading dimension of a staged 16x16 FP64 tile
+
+// Coalesced fetch of the lower block (BI, BJ) of the FP64 matrix: lane
+// (i = lane%16, c = lane/16) reads rows 16BI+i of columns 16BJ+c+4u.
+__device__ __forceinline__ void fetch_block(const double *__restrict__ a, int64_t lda, int n, int BI,
+                                            int BJ, double (&v)[4]) {
+    const int lane = threadIdx.x & 63;
+    const int row = 16 * BI + (lane & 15);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int col = 16 * BJ + (lane >> 4) + 4 * u;
+        v[u] = (row < n && col < n && row >= col) ? a[row + (int64_t)col * lda] : 0.0;
+    }
+}
+
+// FP64 residual R = B - A X for one 4-column chunk.  The wave owning block
+// row I stages the eight blocks of that row (lower blocks (I,J) for J <= I,
+// (J,I) for J > I) one at a time through its private LDS tile with
+// coalesced loads, prefetching the next block into registers.  Lane
+// (a = lane%16, g = lane/16) accumulates row 16I+a over columns
+// 16J+4g..16J+4g+3; two xor shuffles sum the four column quarters.
+// XD is [row][4] FP64 (LDS); b holds this lane's b[16I+a][g] for its block
+// rows; the FP32 residual goes to R (column-major, ld npad).
+__device__ __forceinline__ void residual_fp64(const double *__restrict__ a, int64_t lda, int n, int nb,
+                                              const double *XD, const double (&bv)[2], float *R,
+                                              double *tile, double (*s_red)[3][kRhsChunk],
+                                              double *s_amax) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ai = lane & 15, g = lane >> 4;
+    const int npad = 16 * nb;
+    double rmax = 0.0, xmax = 0.0, bmax = 0.0, amax = 0.0;
+    int slot = 0;
+    for (int I = wave; I < nb; I += kWaves, ++slot) {
+        const int p = 16 * I + ai;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, arow = 0.0;
+        double v[4];
+        fetch_block(a, lda, n, I >= 0 ? I : 0, 0, v);  // block (I,0)
+        for (int J = 0; J < nb; ++J) {
+            // stage block J's lower storage, prefetch block J+1
+#pragma unroll
+            for (int u = 0; u < 4; ++u) tile[ai + kTileLd * ((lane >> 4) + 4 * u)] = v[u];
+            if (J + 1 < nb) {
+                const int Jn = J + 1;
+                fetch_block(a, lda, n, Jn <= I ? I : Jn, Jn <= I ? Jn : I, v);
+            }
+            __builtin_amdgcn_wave_barrier();
+            // element (16I+a, 16J+c): stored at (a,c) of block (I,J) if J < I,
+            // at (c,a) of block (J,I) if J > I, and at (max,min) if J == I
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int c = 4 * g + u;
+                const int r_ = (J < I || (J == I && c <= ai)) ? ai : c;
+                const int c_ = (J < I || (J == I && c <= ai)) ? c : ai;
+                const double av = tile[r_ + kTileLd * c_];
+                const double *x = XD + 4 * (16 * J + c);
+                s0 += av * x[0];
+                s1 += av * x[1];
+                s2 += av * x[2];
+                s3 += av * x[3];
+                arow += fabs(av);
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+#pragma unroll
+        for (int off = 16; off <= 32; off <<= 1) {
+            s0 += __shfl_xor(s0, off, kWave);
+            s1 += __shfl_xor(s1, off, kWave);
+            s2 += __shfl_xor(s2, off, kWave);
+            s3 += __shfl_xor(s3, off, kWave);
+            arow += __shfl_xor(arow, off, kWave);
+        }
+        const double sg = (g == 0) ? s0 : (g == 1) ? s1 : (g == 2) ? s2 : s3;
+        const double b = bv[slot];
+        const double r = b - sg;
+        R[p + npad * g] = (float)r;
+        rmax = fmax(rmax, fabs(r));
+        bmax = fmax(bmax, fabs(b));
+        xmax = fmax(xmax, fabs(XD[4 * p + g]));
+        amax = fmax(amax, arow);
+    }
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+        rmax = fmax(rmax, __shfl_xor(rmax, off, kWave));
+        xmax = fmax(xmax, __shfl_xor(xmax, off, kWave));
+        bmax = fmax(bmax, __shfl_xor(bmax, off, kWave));
+        amax = fmax(amax, __shfl_xor(amax, off, kWave));
+    }
+    if (ai == 0) {
+        s_red[wave][0][g] = rmax;
+        s_red[wave][1][g] = xmax;
+        s_red[wave][2][g] = bmax;
+    }
+    if (lane == 0) s_amax[wave] = amax;
+}
+
+// FP32 factor in LDS, FP64 iterate in LDS, right-hand sides in registers,
+// FP64 residual from the FP64 matrix in HBM/L2.  Stopping test per column:
+// ||b - A x||_inf <= tol (||A||_inf ||x||_inf + ||b||_inf).
 template <int NB>
 __global__ __launch_bounds__(kThreads) void posv_mixed_kernel(
     int n, int nrhs, const double *__restrict__ A, int64_t lda, int64_t strideA,
     const double *__restrict__ B, int64_t ldb, int64_t strideB, double *__restrict__ X,
-    int64_t ldx, int64_t strideX, double tol, int max_iter, int *iters, int64_t batch) {
+    int64_t ldx, int64_t strideX, double tol, int max_iter, int *iters, int *info, int64_t batch) {
+    static_assert(NB <= 2 * kWaves, "two block rows per wavefront at most");
     __shared__ float L[tri(NB) * 256];
-    __shared__ float R[16 * NB * kRhsChunk];   // FP32 right-hand side / correction
-    __shared__ double XD[16 * NB * kRhsChunk];  // FP64 iterate
+    __shared__ float R[16 * NB * kRhsChunk];                                // FP32 rhs / correction, col-major
+    __shared__ __attribute__((aligned(16))) double XD[16 * NB * kRhsChunk];  // FP64 iterate [row][4]
+    __shared__ double TILE[kWaves][16 * kTileLd];                           // per-wave staging tile
     __shared__ double s_red[kWaves][3][kRhsChunk];
     __shared__ double s_amax[kWaves];
     __shared__ int s_flag;
@@ -428,6 +606,7 @@ __global__ __launch_bounds__(kThreads) void posv_mixed_kernel(
     const double *a = A + b * strideA;
     const double *bb = B + b * strideB;
     double *xx = X + b * strideX;
+    if (threadIdx.x == 0) info[b] = 0;
 
     load_lower(a, lda, n, nb, L);
     __syncthreads();
@@ -439,11 +618,27 @@ __global__ __launch_bounds__(kThreads) void posv_mixed_kernel(
     int used = 0;
     for (int r0 = 0; r0 < nrhs; r0 += kRhsChunk) {
         const int nr = (nrhs - r0 < kRhsChunk) ? (nrhs - r0) : kRhsChunk;
+        // b of this lane's rows (block rows wave, wave+4) and column g
+        double bv[2];
+        {
+            const int g = lane >> 4;
+#pragma unroll
+            for (int s2 = 0; s2 < 2; ++s2) {
+                const int p = 16 * (wave + kWaves * s2) + (lane & 15);
+                bv[s2] = (p < n && g < nr) ? bb[p + (int64_t)(r0 + g) * ldb] : 0.0;
+            }
+        }
         __syncthreads();
-        for (int e = threadIdx.x; e < npad * kRhsChunk; e += kThreads) {
-            const int row = e % npad, c = e / npad;
-            R[e] = (row < n && c < nr) ? (float)bb[row + (int64_t)(r0 + c) * ldb] : 0.f;
-            XD[e] = 0.0;
+        {
+            const int g = lane >> 4;
+#pragma unroll
+            for (int s2 = 0; s2 < 2; ++s2) {
+                const int p = 16 * (wave + kWaves * s2) + (lane & 15);
+                if (p < npad) {
+                    R[p + npad * g] = (float)bv[s2];
+                    XD[4 * p + g] = 0.0;
+                }
+            }
         }
         int it = 0;
         bool converged = false;
@@ -451,45 +646,10 @@ __global__ __launch_bounds__(kThreads) void posv_mixed_kernel(
             // correction: solve L L^T d = R (FP32), x += d (FP64)
             __syncthreads();
             solve_blocked(L, nb, R);
-            for (int e = threadIdx.x; e < npad * kRhsChunk; e += kThreads) XD[e] += (double)R[e];
+            for (int e = threadIdx.x; e < npad * kRhsChunk; e += kThreads)
+                XD[e] += (double)R[(e >> 2) + npad * (e & 3)];
             __syncthreads();
-            // FP64 residual r = b - A x, norms per right-hand side
-            const int i = lane & 15, rr = lane >> 4;
-            double rmax = 0.0, xmax = 0.0, bmax = 0.0, amax = 0.0;
-            for (int I = wave; I < nb; I += kWaves) {
-                const int p = 16 * I + i;
-                double acc = 0.0, arow = 0.0;
-                if (p < n) {
-                    acc = (rr < nr) ? bb[p + (int64_t)(r0 + rr) * ldb] : 0.0;
-                    bmax = fmax(bmax, fabs(acc));
-                    for (int q = 0; q < n; ++q) {
-                        const int hi = p > q ? p : q, lo = p > q ? q : p;
-                        const double aq = a[hi + (int64_t)lo * lda];
-                        acc -= aq * XD[q + npad * rr];
-                        arow += fabs(aq);
-                    }
-                    xmax = fmax(xmax, fabs(XD[p + npad * rr]));
-                }
-                R[p + npad * rr] = (float)acc;
-                rmax = fmax(rmax, fabs(acc));
-                amax = fmax(amax, arow);
-            }
-            // reduce over the 16 rows of a lane group, then over waves
-#pragma unroll
-            for (int off = 8; off > 0; off >>= 1) {
-                rmax = fmax(rmax, __shfl_xor(rmax, off, kWave));
-                xmax = fmax(xmax, __shfl_xor(xmax, off, kWave));
-                bmax = fmax(bmax, __shfl_xor(bmax, off, kWave));
-                amax = fmax(amax, __shfl_xor(amax, off, kWave));
-            }
-            amax = fmax(amax, __shfl_xor(amax, 16, kWave));
-            amax = fmax(amax, __shfl_xor(amax, 32, kWave));
-            if (i == 0) {
-                s_red[wave][0][rr] = rmax;
-                s_red[wave][1][rr] = xmax;
-                s_red[wave][2][rr] = bmax;
-            }
-            if (lane == 0) s_amax[wave] = amax;
+            residual_fp64(a, lda, n, nb, XD, bv, R, TILE[wave], s_red, s_amax);
             __syncthreads();
             // every thread evaluates the same test (uniform branch)
             double anorm = 0.0;
@@ -516,9 +676,8 @@ __global__ __launch_bounds__(kThreads) void posv_mixed_kernel(
             used = -1;
             break;
         }
-        __syncthreads();
         for (int e = threadIdx.x; e < npad * kRhsChunk; e += kThreads) {
-            const int row = e % npad, c = e / npad;
+            const int row = e >> 2, c = e & 3;
             if (row < n && c < nr) xx[row + (int64_t)(r0 + c) * ldx] = XD[e];
         }
     }
@@ -661,15 +820,10 @@ int gputils_dposv_mixed_batched(int64_t n, int64_t nrhs, const double *A, int64_
     GPUTILS_ARG(iters != nullptr, 14, "iters is NULL");
     GPUTILS_ARG(info != nullptr, 15, "info is NULL");
     hipStream_t st = as_stream(stream);
-    // info defaults to 0 for matrices that the mixed kernel solves
-    {
-        int rc = gputils_fill_bytes_async(info, 0, sizeof(int) * (size_t)batch, stream);
-        if (rc != GPUTILS_OK) return rc;
-    }
     const dim3 grid((unsigned)batch), block(chol::kThreads);
 #define GPUTILS_MIXED(NBV)                                                                      \
     hipLaunchKernelGGL((chol::posv_mixed_kernel<NBV>), grid, block, 0, st, (int)n, (int)nrhs, A, \
-                       lda, strideA, B, ldb, strideB, X, ldx, strideX, tol, max_iter, iters, batch)
+                       lda, strideA, B, ldb, strideB, X, ldx, strideX, tol, max_iter, iters, info, batch)
     if (n <= 16) GPUTILS_MIXED(1);
     else if (n <= 32) GPUTILS_MIXED(2);
     else if (n <= 64) GPUTILS_MIXED(4);

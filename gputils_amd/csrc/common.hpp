@@ -57,16 +57,19 @@ inline unsigned grid_1d(int64_t work, int per_block, int64_t cap = 65536) {
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-// 16x16x4 MFMA in float or double.  Operand layout (CDNA3/CDNA4 ISA):
+// 16x16x4 MFMA in float or double.  Operand layout (measured on gfx950):
 //   A (16x4):  lane l holds A[l % 16][l / 16]
 //   B (4x16):  lane l holds B[l / 16][l % 16]
-//   D (16x16): lane l, item r holds D[4 * (l / 16) + r][l % 16]
+//   D (16x16): lane l, item r holds D[drow(l, r)][l % 16] with
+//              f32: drow = 4 * (l / 16) + r      (rows blocked by lane group)
+//              f64: drow = (l / 16) + 4 * r      (rows interleaved)
 template <typename T>
 struct Mfma16;
 
 template <>
 struct Mfma16<float> {
     using acc_t = f32x4;
+    __device__ static inline int drow(int lane, int r) { return 4 * (lane >> 4) + r; }
     __device__ static inline acc_t zero() { return acc_t{0.f, 0.f, 0.f, 0.f}; }
     __device__ static inline acc_t fma(float a, float b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
@@ -76,6 +79,7 @@ struct Mfma16<float> {
 template <>
 struct Mfma16<double> {
     using acc_t = f64x4;
+    __device__ static inline int drow(int lane, int r) { return (lane >> 4) + 4 * r; }
     __device__ static inline acc_t zero() { return acc_t{0.0, 0.0, 0.0, 0.0}; }
     __device__ static inline acc_t fma(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);

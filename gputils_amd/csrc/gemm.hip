@@ -11,7 +11,7 @@
 //   float, BF16X3 mode v_mfma_f32_16x16x32_bf16 x3: a = ah + al, b = bh + bl
 //                      (bf16 terms), C += ah bh + ah bl + al bh, fp32 acc.
 //   double             v_mfma_f64_16x16x4_f64
-// The C tile is staged through LDS and written with 16-byte stores.
+// The C tile is staged through LDS and written column by column (coalesced).
 //
 // Per-matrix path (C aliasing B, or very narrow C): one workgroup per
 // matrix stages all of B_i in LDS before any C_i element is written, which
@@ -144,14 +144,14 @@ __global__ __launch_bounds__(kThreads) void gemm_tiled_kernel(int m, int n, int 
     __syncthreads();
     T *Cs = smem;
     {
-        const int g = lane >> 4, col = lane & 15;
+        const int col = lane & 15;
 #pragma unroll
         for (int x = 0; x < 2; ++x)
 #pragma unroll
             for (int y = 0; y < 2; ++y)
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
-                    Cs[(wm + 16 * x + 4 * g + r) + (wn + 16 * y + col) * (kTM + 1)] = acc[x][y][r];
+                    Cs[(wm + 16 * x + M::drow(lane, r)) + (wn + 16 * y + col) * (kTM + 1)] = acc[x][y][r];
     }
     __syncthreads();
     for (int e = threadIdx.x; e < kTM * kTN; e += kThreads) {
