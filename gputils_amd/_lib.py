@@ -22,6 +22,7 @@ _CTYPES = {
     "float": ctypes.c_float,
     "size_t": ctypes.c_size_t,
     "unsigned char": ctypes.c_ubyte,
+    "gputils_gels_plan_t": ctypes.c_void_p,
     "void": None,
 }
 

@@ -203,3 +203,85 @@ def transpose_batched(A: torch.Tensor) -> torch.Tensor:
     check(getattr(lib(), f"gputils_{_pfx(A)}transpose_batched")(m, n, _ptr(A), m, m * n, _ptr(B), n, m * n, k,
                                                                  stream_handle()), "transpose_batched")
     return B
+
+
+# ------------------------------------------------------------ QR / LS --
+def gels_batched(A: torch.Tensor, B: torch.Tensor, info: torch.Tensor | None = None) -> torch.Tensor:
+    """Least squares for A (k, n, m) storage of m x n (m >= n) and B (k, nrhs, m):
+    A <- QR factors, B[:, :, :n] <- solution."""
+    _check_dev(A, "A")
+    _check_dev(B, "B")
+    k, n, m = A.shape
+    nrhs = B.shape[1]
+    info = _info(k, A.device, info)
+    check(getattr(lib(), f"gputils_{_pfx(A)}gels_batched")(m, n, nrhs, _ptr(A), m, m * n, _ptr(B), m, m * nrhs,
+                                                          _ptr(info), k, stream_handle()), "gels_batched")
+    return info
+
+
+def geqrf_batched(A: torch.Tensor) -> torch.Tensor:
+    """Householder QR in place (LAPACK storage); returns tau (k, n)."""
+    _check_dev(A, "A")
+    k, n, m = A.shape
+    tau = torch.empty((k, n), dtype=A.dtype, device=A.device)
+    check(getattr(lib(), f"gputils_{_pfx(A)}geqrf_batched")(m, n, _ptr(A), m, m * n, _ptr(tau), n, None, k,
+                                                           stream_handle()), "geqrf_batched")
+    return tau
+
+
+def ormqr_batched(QR: torch.Tensor, tau: torch.Tensor, C: torch.Tensor, trans: bool = True) -> torch.Tensor:
+    """C (k, ncols, m) <- Q^T C (trans) or Q C, Q from geqrf_batched."""
+    _check_dev(QR, "QR")
+    _check_dev(C, "C")
+    k, n, m = QR.shape
+    ncols = C.shape[1]
+    check(getattr(lib(), f"gputils_{_pfx(QR)}ormqr_batched")(1 if trans else 0, m, ncols, n, _ptr(QR), m, m * n,
+                                                            _ptr(tau), n, _ptr(C), m, m * ncols, k,
+                                                            stream_handle()), "ormqr_batched")
+    return C
+
+
+def trsm_upper_batched(R: torch.Tensor, B: torch.Tensor, ldr: int | None = None) -> torch.Tensor:
+    """B (k, nrhs, ldb>=n) <- R^{-1} B for the leading n x n upper triangle of R
+    (storage (k, n, ldr))."""
+    _check_dev(R, "R")
+    _check_dev(B, "B")
+    k, n, ld = R.shape
+    nrhs, ldb = B.shape[1], B.shape[2]
+    check(getattr(lib(), f"gputils_{_pfx(R)}trsm_upper_batched")(n, nrhs, _ptr(R), ld, ld * n, _ptr(B), ldb,
+                                                                ldb * nrhs, k, stream_handle()), "trsm_upper")
+    return B
+
+
+class RaggedLeastSquares:
+    """Plan for many least-squares systems of uneven size (FP32 QR + one FP64
+    refinement).  System s: m[s] x n[s] column-major at A_flat[offA[s]:],
+    right-hand side at B_flat[offB[s]:]; the solution overwrites B[offB[s]:][:n[s]]."""
+
+    def __init__(self, m, n, offA, offB):
+        import numpy as np
+        self._m = np.ascontiguousarray(m, dtype=np.int32)
+        self._n = np.ascontiguousarray(n, dtype=np.int32)
+        self._oa = np.ascontiguousarray(offA, dtype=np.int64)
+        self._ob = np.ascontiguousarray(offB, dtype=np.int64)
+        self._plan = ctypes.c_void_p()
+        check(lib().gputils_gels_ragged_plan_create(ctypes.byref(self._plan), len(self._m),
+                                                    self._m.ctypes.data_as(ctypes.c_void_p),
+                                                    self._n.ctypes.data_as(ctypes.c_void_p),
+                                                    self._oa.ctypes.data_as(ctypes.c_void_p),
+                                                    self._ob.ctypes.data_as(ctypes.c_void_p)), "plan_create")
+
+    def __call__(self, A_flat: torch.Tensor, B_flat: torch.Tensor, refine_steps: int = 2) -> torch.Tensor:
+        _check_dev(A_flat, "A", (torch.float64,))
+        _check_dev(B_flat, "B", (torch.float64,))
+        check(lib().gputils_dgels_ragged(self._plan, _ptr(A_flat), _ptr(B_flat), refine_steps, stream_handle()),
+              "gels_ragged")
+        return B_flat
+
+    def __del__(self):
+        try:
+            if self._plan:
+                lib().gputils_gels_ragged_plan_destroy(self._plan)
+                self._plan = ctypes.c_void_p()
+        except Exception:
+            pass

@@ -1,0 +1,606 @@
+// qr.hip -- batched Householder QR, Q^T application, triangular solve and
+// least squares for gfx950.
+//
+// Replaces cublas?gelsBatched behind DTensor::leastSquaresBatched
+// (include/tensor.cuh:1340-1394) and cusolverDn?geqrf / ?ormqr / cublas?trsm
+// behind QRFactoriser (tensor.cuh:1796-1995), and adds a ragged
+// mixed-precision least-squares solver (uneven m_i x n_i systems in one
+// plan: FP32 Householder QR with FP64 norms and dot products plus one FP64
+// refinement step).
+//
+// One 256-thread workgroup per matrix.  The matrix lives in LDS when it
+// fits (otherwise the kernel works on it in global memory).  Column c is
+// owned by wavefront c % 4.  Step j: every wave applies reflector j to the
+// columns it owns; the owner of column j+1 does that column first and then
+// forms reflector j+1 (dlarfg convention: beta = -sign(alpha)*||x||,
+// tau = (beta - alpha)/beta, v = x / (alpha - beta)), so one barrier per
+// column suffices.
+#include <algorithm>
+#include <vector>
+
+#include "common.hpp"
+
+namespace gputils {
+namespace qr {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / kWave;
+
+// Order one wavefront's earlier memory writes before its later reads by
+// other lanes (LDS or global): a wavefront-scope fence plus a scheduling
+// barrier.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Reflector of column j (one wavefront).  Scales v in place, writes beta to
+// A[j][j] and returns tau (identical in every lane).
+template <typename T, typename ACC>
+__device__ __forceinline__ T make_reflector(T *A, int64_t lda, int m, int j) {
+    const int lane = threadIdx.x & 63;
+    T *col = A + (int64_t)j * lda;
+    ACC s = ACC(0);
+    for (int i = j + 1 + lane; i < m; i += kWave) {
+        const ACC x = (ACC)col[i];
+        s += x * x;
+    }
+    s = wave_sum(s);
+    const ACC alpha = (ACC)col[j];
+    if (s == ACC(0)) return T(0);  // H = I (dlarfg with x = 0)
+    const ACC norm = sqrt(alpha * alpha + s);
+    const ACC beta = (alpha >= ACC(0)) ? -norm : norm;
+    const ACC tau = (beta - alpha) / beta;
+    const T scale = (T)(ACC(1) / (alpha - beta));
+    for (int i = j + 1 + lane; i < m; i += kWave) col[i] *= scale;
+    wave_sync();
+    if (lane == 0) col[j] = (T)beta;
+    return (T)tau;
+}
+
+// x <- (I - tau v v^T) x on rows j..m-1 of column c, v = [1; A[j+1:m, j]]
+// (one wavefront).
+template <typename T, typename ACC>
+__device__ __forceinline__ void apply_reflector_col(T *A, int64_t lda, int m, int j, T tau, int c) {
+    const int lane = threadIdx.x & 63;
+    const T *v = A + (int64_t)j * lda;
+    T *x = A + (int64_t)c * lda;
+    ACC w = ACC(0);
+    for (int i = j + 1 + lane; i < m; i += kWave) w += (ACC)v[i] * (ACC)x[i];
+    w = wave_sum(w) + (ACC)x[j];
+    const T tw = (T)((ACC)tau * w);
+    for (int i = j + 1 + lane; i < m; i += kWave) x[i] -= tw * v[i];
+    wave_sync();
+    if (lane == 0) x[j] -= tw;
+}
+
+// Householder QR of the m x n (m >= n) matrix at A (LDS or global), tau[j]
+// for j < n.  s_tau: two LDS slots.  Ends with a barrier.
+template <typename T, typename ACC>
+__device__ __forceinline__ void hqr_factor(T *A, int64_t lda, int m, int n, T *tau, T *s_tau) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (n <= 0) return;
+    if (wave == 0) {
+        const T t = make_reflector<T, ACC>(A, lda, m, 0);
+        if (lane == 0) {
+            s_tau[0] = t;
+            tau[0] = t;
+        }
+    }
+    __syncthreads();
+    for (int j = 0; j < n; ++j) {
+        const T t = s_tau[j & 1];
+        if (j + 1 < n && wave == ((j + 1) & (kWaves - 1))) {
+            apply_reflector_col<T, ACC>(A, lda, m, j, t, j + 1);
+            const T t1 = make_reflector<T, ACC>(A, lda, m, j + 1);
+            if (lane == 0) {
+                s_tau[(j + 1) & 1] = t1;
+                tau[j + 1] = t1;
+            }
+        }
+        if (t != T(0)) {
+            for (int c = j + 2 + ((wave - (j + 2)) & (kWaves - 1)); c < n; c += kWaves)
+                apply_reflector_col<T, ACC>(A, lda, m, j, t, c);
+        }
+        __syncthreads();
+    }
+}
+
+// c <- Q^T c (trans) or Q c for one m-vector in LDS, one wavefront; Q from
+// the first k reflectors stored in A / tau.
+template <typename T, typename ACC>
+__device__ __forceinline__ void apply_q_vec(const T *A, int64_t lda, int m, int k, const T *tau, T *c,
+                                            bool trans) {
+    const int lane = threadIdx.x & 63;
+    for (int t = 0; t < k; ++t) {
+        const int j = trans ? t : (k - 1 - t);
+        const T tj = tau[j];
+        if (tj == T(0)) continue;
+        const T *v = A + (int64_t)j * lda;
+        ACC w = ACC(0);
+        for (int i = j + 1 + lane; i < m; i += kWave) w += (ACC)v[i] * (ACC)c[i];
+        w = wave_sum(w) + (ACC)c[j];
+        const T tw = (T)((ACC)tj * w);
+        for (int i = j + 1 + lane; i < m; i += kWave) c[i] -= tw * v[i];
+        wave_sync();
+        if (lane == 0) c[j] -= tw;
+        wave_sync();
+    }
+}
+
+// Solve R y = c in place (R upper triangular n x n at A, c in LDS), one
+// wavefront, column-oriented back substitution.
+template <typename T>
+__device__ __forceinline__ void trsv_upper(const T *A, int64_t lda, int n, T *c) {
+    const int lane = threadIdx.x & 63;
+    for (int k = n - 1; k >= 0; --k) {
+        const T ck = c[k] / A[k + (int64_t)k * lda];
+        for (int i = lane; i < k; i += kWave) c[i] -= A[i + (int64_t)k * lda] * ck;
+        wave_sync();
+        if (lane == 0) c[k] = ck;
+        wave_sync();
+    }
+}
+
+template <typename T>
+struct Acc;
+template <>
+struct Acc<float> {
+    using type = float;
+};
+template <>
+struct Acc<double> {
+    using type = double;
+};
+
+// ------------------------------------------------------------- kernels --
+// Uniform batched least squares (gelsBatched): A_b <- QR factors,
+// B_b[0:n, :] <- x.  MAXE > 0: matrix staged in LDS (m*n <= MAXE).
+template <typename T, int MAXE>
+__global__ __launch_bounds__(kThreads) void gels_kernel(int m, int n, int nrhs, T *A, int64_t lda,
+                                                        int64_t strideA, T *B, int64_t ldb,
+                                                        int64_t strideB, T *tauws, int *info,
+                                                        int64_t batch) {
+    using ACC = typename Acc<T>::type;
+    constexpr int kLds = MAXE > 0 ? MAXE : 1;
+    __shared__ T S[kLds];
+    __shared__ T s_tau[2];
+    extern __shared__ unsigned char dyn[];  // tau (n) and one right-hand side (m)
+    T *tau = reinterpret_cast<T *>(dyn);
+    T *c = tau + n;
+    const int64_t b = blockIdx.x;
+    if (b >= batch) return;
+    T *a = A + b * strideA;
+    T *W = (MAXE > 0) ? S : a;
+    const int64_t ldw = (MAXE > 0) ? m : lda;
+    if (MAXE > 0) {
+        for (int e = threadIdx.x; e < m * n; e += kThreads) S[e] = a[(e % m) + (int64_t)(e / m) * lda];
+        __syncthreads();
+    }
+    hqr_factor<T, ACC>(W, ldw, m, n, tau, s_tau);
+    if (MAXE > 0) {
+        for (int e = threadIdx.x; e < m * n; e += kThreads) a[(e % m) + (int64_t)(e / m) * lda] = S[e];
+    }
+    const int wave = threadIdx.x >> 6;
+    T *bb = B + b * strideB;
+    for (int r = 0; r < nrhs; ++r) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < m; i += kThreads) c[i] = bb[i + (int64_t)r * ldb];
+        __syncthreads();
+        if (wave == 0) {
+            apply_q_vec<T, ACC>(W, ldw, m, n, tau, c, true);
+            trsv_upper<T>(W, ldw, n, c);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += kThreads) bb[i + (int64_t)r * ldb] = c[i];
+    }
+    if (tauws != nullptr)
+        for (int i = threadIdx.x; i < n; i += kThreads) tauws[b * n + i] = tau[i];
+    if (threadIdx.x == 0 && info != nullptr) info[b] = 0;
+}
+
+// Batched QR factorisation only (geqrf): A_b <- QR factors, tau_b.
+template <typename T, int MAXE>
+__global__ __launch_bounds__(kThreads) void geqrf_kernel(int m, int n, T *A, int64_t lda, int64_t strideA,
+                                                         T *tau, int64_t strideTau, int *info,
+                                                         int64_t batch) {
+    using ACC = typename Acc<T>::type;
+    constexpr int kLds = MAXE > 0 ? MAXE : 1;
+    __shared__ T S[kLds];
+    __shared__ T s_tau[2];
+    const int64_t b = blockIdx.x;
+    if (b >= batch) return;
+    T *a = A + b * strideA;
+    T *W = (MAXE > 0) ? S : a;
+    const int64_t ldw = (MAXE > 0) ? m : lda;
+    if (MAXE > 0) {
+        for (int e = threadIdx.x; e < m * n; e += kThreads) S[e] = a[(e % m) + (int64_t)(e / m) * lda];
+        __syncthreads();
+    }
+    hqr_factor<T, ACC>(W, ldw, m, n, tau + b * strideTau, s_tau);
+    if (MAXE > 0)
+        for (int e = threadIdx.x; e < m * n; e += kThreads) a[(e % m) + (int64_t)(e / m) * lda] = S[e];
+    if (threadIdx.x == 0 && info != nullptr) info[b] = 0;
+}
+
+// C_b <- Q_b^T C_b (trans) or Q_b C_b; each wavefront takes columns of C.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void ormqr_kernel(int trans, int m, int ncols, int k, const T *A,
+                                                         int64_t lda, int64_t strideA, const T *tau,
+                                                         int64_t strideTau, T *C, int64_t ldc,
+                                                         int64_t strideC, int64_t batch) {
+    using ACC = typename Acc<T>::type;
+    extern __shared__ unsigned char dyn[];
+    T *cw = reinterpret_cast<T *>(dyn) + (threadIdx.x >> 6) * m;  // one column per wave
+    const int64_t b = blockIdx.x;
+    if (b >= batch) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const T *a = A + b * strideA;
+    const T *t = tau + b * strideTau;
+    T *cb = C + b * strideC;
+    for (int col = wave; col < ncols; col += kWaves) {
+        for (int i = lane; i < m; i += kWave) cw[i] = cb[i + (int64_t)col * ldc];
+        wave_sync();
+        apply_q_vec<T, ACC>(a, lda, m, k, t, cw, trans != 0);
+        for (int i = lane; i < m; i += kWave) cb[i + (int64_t)col * ldc] = cw[i];
+        wave_sync();
+    }
+}
+
+// B_b <- R_b^{-1} B_b, R upper triangular n x n (non-unit).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void trsm_upper_kernel(int n, int nrhs, const T *R, int64_t ldr,
+                                                              int64_t strideR, T *B, int64_t ldb,
+                                                              int64_t strideB, int64_t batch) {
+    extern __shared__ unsigned char dyn[];
+    T *cw = reinterpret_cast<T *>(dyn) + (threadIdx.x >> 6) * n;
+    const int64_t b = blockIdx.x;
+    if (b >= batch) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const T *r = R + b * strideR;
+    T *bb = B + b * strideB;
+    for (int col = wave; col < nrhs; col += kWaves) {
+        for (int i = lane; i < n; i += kWave) cw[i] = bb[i + (int64_t)col * ldb];
+        wave_sync();
+        trsv_upper<T>(r, ldr, n, cw);
+        for (int i = lane; i < n; i += kWave) bb[i + (int64_t)col * ldb] = cw[i];
+        wave_sync();
+    }
+}
+
+// -------------------------------------------------- ragged, mixed --
+struct RaggedItem {
+    int m, n;
+    int64_t offA, offB;
+};
+
+// Solve R^T y = c in place (forward substitution), one wavefront.
+template <typename T>
+__device__ __forceinline__ void trsv_upper_trans(const T *A, int64_t lda, int n, T *c) {
+    const int lane = threadIdx.x & 63;
+    for (int k = 0; k < n; ++k) {
+        const T ck = c[k] / A[k + (int64_t)k * lda];
+        for (int i = k + 1 + lane; i < n; i += kWave) c[i] -= A[k + (int64_t)i * lda] * ck;
+        wave_sync();
+        if (lane == 0) c[k] = ck;
+        wave_sync();
+    }
+}
+
+// One bucket of the ragged plan: a persistent grid walks the cost-sorted
+// item list with a static stride.  Per system: FP64 A -> FP32 (LDS), FP32
+// Householder QR with FP64 norms/dots, x0 = R^{-1} Q^T b, r0 = b - A x0 in
+// FP64, then `steps` refinements of the augmented system
+// [I A; A^T 0][r; x] = [b; 0] (Bjorck 1967): f = b - r - A x, g = -A^T r
+// in FP64, and with the FP32 factors h = R^{-T} g, d = Q^T f,
+// dx = R^{-1}(d[0:n] - h), dr = Q [h; d[n:m]].  This recovers FP64
+// accuracy also for inconsistent systems, where refining x alone stalls at
+// the FP32 level.  x overwrites b[0:n].
+template <int MAXE>
+__global__ __launch_bounds__(kThreads) void gels_ragged_kernel(const RaggedItem *__restrict__ items,
+                                                               int nitems, const double *__restrict__ A,
+                                                               double *B, int steps) {
+    __shared__ float S[MAXE];
+    __shared__ float s_tau[2];
+    __shared__ float tau[64];
+    __shared__ float c[512];
+    __shared__ float h[64];
+    __shared__ double xd[64];
+    __shared__ double rd[512];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const RaggedItem w = items[it];
+        const int m = w.m, n = w.n;
+        const double *a = A + w.offA;
+        double *bb = B + w.offB;
+        __syncthreads();
+        for (int e = threadIdx.x; e < m * n; e += kThreads) S[e] = (float)a[e];
+        for (int i = threadIdx.x; i < m; i += kThreads) c[i] = (float)bb[i];
+        __syncthreads();
+        hqr_factor<float, double>(S, m, m, n, tau, s_tau);
+        if (wave == 0) {
+            apply_q_vec<float, double>(S, m, m, n, tau, c, true);
+            trsv_upper<float>(S, m, n, c);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += kThreads) xd[i] = (double)c[i];
+        __syncthreads();
+        // r0 = b - A x0 (FP64, A from global memory; coalesced over rows)
+        for (int i = threadIdx.x; i < m; i += kThreads) {
+            double r = bb[i];
+            for (int k = 0; k < n; ++k) r -= a[i + (int64_t)k * m] * xd[k];
+            rd[i] = r;
+        }
+        for (int s = 0; s < steps; ++s) {
+            __syncthreads();
+            // f = b - r - A x  -> c (FP32)
+            for (int i = threadIdx.x; i < m; i += kThreads) {
+                double f = bb[i] - rd[i];
+                for (int k = 0; k < n; ++k) f -= a[i + (int64_t)k * m] * xd[k];
+                c[i] = (float)f;
+            }
+            // g = -A^T r -> h (FP32); one wavefront per column
+            for (int j = wave; j < n; j += kWaves) {
+                double g = 0.0;
+                for (int i = lane; i < m; i += kWave) g += a[i + (int64_t)j * m] * rd[i];
+                g = wave_sum(g);
+                if (lane == 0) h[j] = (float)(-g);
+            }
+            __syncthreads();
+            if (wave == 0) {
+                trsv_upper_trans<float>(S, m, n, h);                 // h = R^{-T} g
+                apply_q_vec<float, double>(S, m, m, n, tau, c, true);  // c = Q^T f
+                for (int i = lane; i < n; i += kWave) c[i] -= h[i];
+                wave_sync();
+                trsv_upper<float>(S, m, n, c);                       // c[0:n] = dx
+                for (int i = lane; i < n; i += kWave) {
+                    xd[i] += (double)c[i];
+                    c[i] = h[i];
+                }
+                wave_sync();
+                apply_q_vec<float, double>(S, m, m, n, tau, c, false); // c = dr = Q [h; d2]
+            }
+            __syncthreads();
+            for (int i = threadIdx.x; i < m; i += kThreads) rd[i] += (double)c[i];
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += kThreads) bb[i] = xd[i];
+    }
+}
+
+template <typename T>
+int gels_dispatch(int64_t m, int64_t n, int64_t nrhs, T *A, int64_t lda, int64_t strideA, T *B,
+                  int64_t ldb, int64_t strideB, T *tauws, int *info, int64_t batch, hipStream_t st) {
+    const size_t dyn = sizeof(T) * (size_t)(n + m);
+    if (dyn > 48 * 1024) {
+        set_error("gels: m + n too large for the vector workspace");
+        return -1;
+    }
+    const dim3 grid((unsigned)batch), block(kThreads);
+    const int64_t mn = m * n;
+#define GPUTILS_GELS(ME)                                                                                  \
+    hipLaunchKernelGGL((gels_kernel<T, ME>), grid, block, dyn, st, (int)m, (int)n, (int)nrhs, A, lda, strideA, \
+                       B, ldb, strideB, tauws, info, batch)
+    constexpr int kBig = (int)(56 * 1024 / sizeof(T));
+    if (mn <= 1024) GPUTILS_GELS(1024);
+    else if (mn <= 4096) GPUTILS_GELS(4096);
+    else if (mn <= kBig) GPUTILS_GELS(kBig);
+    else GPUTILS_GELS(0);
+#undef GPUTILS_GELS
+    return launch_check("gels_kernel");
+}
+
+template <typename T>
+int geqrf_dispatch(int64_t m, int64_t n, T *A, int64_t lda, int64_t strideA, T *tau, int64_t strideTau,
+                   int *info, int64_t batch, hipStream_t st) {
+    const dim3 grid((unsigned)batch), block(kThreads);
+    const int64_t mn = m * n;
+#define GPUTILS_GEQRF(ME) \
+    hipLaunchKernelGGL((geqrf_kernel<T, ME>), grid, block, 0, st, (int)m, (int)n, A, lda, strideA, tau, strideTau, info, batch)
+    constexpr int kBig = (int)(56 * 1024 / sizeof(T));
+    if (mn <= 1024) GPUTILS_GEQRF(1024);
+    else if (mn <= 4096) GPUTILS_GEQRF(4096);
+    else if (mn <= kBig) GPUTILS_GEQRF(kBig);
+    else GPUTILS_GEQRF(0);
+#undef GPUTILS_GEQRF
+    return launch_check("geqrf_kernel");
+}
+
+}  // namespace qr
+}  // namespace gputils
+
+using namespace gputils;
+
+// ---------------------------------------------------------- ragged plan --
+struct gputils_gels_plan {
+    int64_t count = 0;
+    qr::RaggedItem *d_items = nullptr;
+    struct Bucket {
+        int maxe;
+        int begin, count;
+        int grid;
+    };
+    std::vector<Bucket> buckets;
+};
+
+extern "C" {
+
+int gputils_sgels_batched(int64_t m, int64_t n, int64_t nrhs, float *A, int64_t lda, int64_t strideA,
+                          float *B, int64_t ldb, int64_t strideB, int *info, int64_t batch, void *stream) {
+    GPUTILS_ARG(m >= 0, 1, "m < 0");
+    GPUTILS_ARG(n >= 0 && n <= m, 2, "need 0 <= n <= m");
+    GPUTILS_ARG(nrhs >= 0, 3, "nrhs < 0");
+    GPUTILS_ARG(lda >= (m > 1 ? m : 1), 5, "lda < max(1,m)");
+    GPUTILS_ARG(ldb >= (m > 1 ? m : 1), 8, "ldb < max(1,m)");
+    GPUTILS_ARG(batch >= 0 && batch <= 0x7fffffff, 11, "batch out of range");
+    if (m == 0 || n == 0 || batch == 0) return GPUTILS_OK;
+    GPUTILS_ARG(A != nullptr, 4, "A is NULL");
+    GPUTILS_ARG(nrhs == 0 || B != nullptr, 7, "B is NULL");
+    return qr::gels_dispatch<float>(m, n, nrhs, A, lda, strideA, B, ldb, strideB, nullptr, info, batch,
+                                    as_stream(stream));
+}
+
+int gputils_dgels_batched(int64_t m, int64_t n, int64_t nrhs, double *A, int64_t lda, int64_t strideA,
+                          double *B, int64_t ldb, int64_t strideB, int *info, int64_t batch, void *stream) {
+    GPUTILS_ARG(m >= 0, 1, "m < 0");
+    GPUTILS_ARG(n >= 0 && n <= m, 2, "need 0 <= n <= m");
+    GPUTILS_ARG(nrhs >= 0, 3, "nrhs < 0");
+    GPUTILS_ARG(lda >= (m > 1 ? m : 1), 5, "lda < max(1,m)");
+    GPUTILS_ARG(ldb >= (m > 1 ? m : 1), 8, "ldb < max(1,m)");
+    GPUTILS_ARG(batch >= 0 && batch <= 0x7fffffff, 11, "batch out of range");
+    if (m == 0 || n == 0 || batch == 0) return GPUTILS_OK;
+    GPUTILS_ARG(A != nullptr, 4, "A is NULL");
+    GPUTILS_ARG(nrhs == 0 || B != nullptr, 7, "B is NULL");
+    return qr::gels_dispatch<double>(m, n, nrhs, A, lda, strideA, B, ldb, strideB, nullptr, info, batch,
+                                     as_stream(stream));
+}
+
+#define GPUTILS_GEQRF_ENTRY(NAME, T)                                                                    \
+    int NAME(int64_t m, int64_t n, T *A, int64_t lda, int64_t strideA, T *tau, int64_t strideTau,       \
+             int *info, int64_t batch, void *stream) {                                                  \
+        GPUTILS_ARG(m >= 0, 1, "m < 0");                                                                \
+        GPUTILS_ARG(n >= 0 && n <= m, 2, "need 0 <= n <= m");                                           \
+        GPUTILS_ARG(lda >= (m > 1 ? m : 1), 4, "lda < max(1,m)");                                       \
+        GPUTILS_ARG(batch >= 0 && batch <= 0x7fffffff, 9, "batch out of range");                        \
+        if (m == 0 || n == 0 || batch == 0) return GPUTILS_OK;                                          \
+        GPUTILS_ARG(A != nullptr, 3, "A is NULL");                                                      \
+        GPUTILS_ARG(tau != nullptr, 6, "tau is NULL");                                                  \
+        return qr::geqrf_dispatch<T>(m, n, A, lda, strideA, tau, strideTau, info, batch, as_stream(stream)); \
+    }
+GPUTILS_GEQRF_ENTRY(gputils_sgeqrf_batched, float)
+GPUTILS_GEQRF_ENTRY(gputils_dgeqrf_batched, double)
+
+#define GPUTILS_ORMQR_ENTRY(NAME, T)                                                                     \
+    int NAME(int trans, int64_t m, int64_t ncols, int64_t k, const T *A, int64_t lda, int64_t strideA,  \
+             const T *tau, int64_t strideTau, T *C, int64_t ldc, int64_t strideC, int64_t batch,        \
+             void *stream) {                                                                             \
+        GPUTILS_ARG(trans == 0 || trans == 1, 1, "trans must be 0 or 1");                               \
+        GPUTILS_ARG(m >= 0, 2, "m < 0");                                                                 \
+        GPUTILS_ARG(ncols >= 0, 3, "ncols < 0");                                                         \
+        GPUTILS_ARG(k >= 0 && k <= m, 4, "need 0 <= k <= m");                                            \
+        GPUTILS_ARG(lda >= (m > 1 ? m : 1), 6, "lda < max(1,m)");                                        \
+        GPUTILS_ARG(ldc >= (m > 1 ? m : 1), 11, "ldc < max(1,m)");                                       \
+        GPUTILS_ARG(batch >= 0 && batch <= 0x7fffffff, 13, "batch out of range");                        \
+        GPUTILS_ARG(sizeof(T) * (size_t)m * 4 <= 64 * 1024, 2, "m too large");                           \
+        if (m == 0 || ncols == 0 || k == 0 || batch == 0) return GPUTILS_OK;                             \
+        hipLaunchKernelGGL((qr::ormqr_kernel<T>), dim3((unsigned)batch), dim3(qr::kThreads),             \
+                           sizeof(T) * (size_t)m * qr::kWaves, as_stream(stream), trans, (int)m,         \
+                           (int)ncols, (int)k, A, lda, strideA, tau, strideTau, C, ldc, strideC, batch); \
+        return launch_check("ormqr_kernel");                                                             \
+    }
+GPUTILS_ORMQR_ENTRY(gputils_sormqr_batched, float)
+GPUTILS_ORMQR_ENTRY(gputils_dormqr_batched, double)
+
+#define GPUTILS_TRSM_ENTRY(NAME, T)                                                                      \
+    int NAME(int64_t n, int64_t nrhs, const T *R, int64_t ldr, int64_t strideR, T *B, int64_t ldb,      \
+             int64_t strideB, int64_t batch, void *stream) {                                             \
+        GPUTILS_ARG(n >= 0, 1, "n < 0");                                                                 \
+        GPUTILS_ARG(nrhs >= 0, 2, "nrhs < 0");                                                           \
+        GPUTILS_ARG(ldr >= (n > 1 ? n : 1), 4, "ldr < max(1,n)");                                        \
+        GPUTILS_ARG(ldb >= (n > 1 ? n : 1), 7, "ldb < max(1,n)");                                        \
+        GPUTILS_ARG(batch >= 0 && batch <= 0x7fffffff, 9, "batch out of range");                         \
+        GPUTILS_ARG(sizeof(T) * (size_t)n * 4 <= 64 * 1024, 1, "n too large");                           \
+        if (n == 0 || nrhs == 0 || batch == 0) return GPUTILS_OK;                                        \
+        hipLaunchKernelGGL((qr::trsm_upper_kernel<T>), dim3((unsigned)batch), dim3(qr::kThreads),        \
+                           sizeof(T) * (size_t)n * qr::kWaves, as_stream(stream), (int)n, (int)nrhs, R,  \
+                           ldr, strideR, B, ldb, strideB, batch);                                        \
+        return launch_check("trsm_upper_kernel");                                                        \
+    }
+GPUTILS_TRSM_ENTRY(gputils_strsm_upper_batched, float)
+GPUTILS_TRSM_ENTRY(gputils_dtrsm_upper_batched, double)
+
+int gputils_gels_ragged_plan_create(gputils_gels_plan_t *plan, int64_t count, const int32_t *m,
+                                    const int32_t *n, const int64_t *offA, const int64_t *offB) {
+    GPUTILS_ARG(plan != nullptr, 1, "plan is NULL");
+    GPUTILS_ARG(count >= 0 && count <= 0x7fffffff, 2, "count out of range");
+    GPUTILS_ARG(count == 0 || (m && n && offA && offB), 3, "size/offset arrays are NULL");
+    *plan = nullptr;
+    // buckets by LDS need (FP32 m*n elements); items sorted by cost m*n^2, largest first
+    static const int kBucketMax[] = {2048, 8192, 32768};
+    std::vector<qr::RaggedItem> items((size_t)count);
+    std::vector<int> order((size_t)count);
+    for (int64_t i = 0; i < count; ++i) {
+        if (m[i] < 1 || n[i] < 1 || n[i] > m[i] || n[i] > 64 || m[i] > 512) {
+            set_error("invalid argument 3: system %lld has m=%d n=%d (need 1 <= n <= m, n <= 64, m <= 512)",
+                      (long long)i, m[i], n[i]);
+            return -3;
+        }
+        order[i] = (int)i;
+    }
+    auto bucket_of = [&](int i) {
+        const int64_t e = (int64_t)m[i] * n[i];
+        return e <= kBucketMax[0] ? 0 : (e <= kBucketMax[1] ? 1 : 2);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        const int bx = bucket_of(x), by = bucket_of(y);
+        if (bx != by) return bx > by;
+        return (int64_t)m[x] * n[x] * n[x] > (int64_t)m[y] * n[y] * n[y];
+    });
+    auto *p = new gputils_gels_plan();
+    p->count = count;
+    int dev = 0, cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+        cus = prop.multiProcessorCount;
+    int start = 0;
+    for (int bk = 2; bk >= 0; --bk) {
+        int cnt = 0;
+        while (start + cnt < count && bucket_of(order[start + cnt]) == bk) ++cnt;
+        if (cnt > 0) {
+            const int lds = kBucketMax[bk] * 4 + 3 * 1024;
+            const int per_cu = std::max(1, std::min(8, 160 * 1024 / lds));
+            p->buckets.push_back({kBucketMax[bk], start, cnt, std::min(cnt, cus * per_cu)});
+        }
+        start += cnt;
+    }
+    for (int64_t i = 0; i < count; ++i) {
+        const int o = order[i];
+        items[i] = {m[o], n[o], offA[o], offB[o]};
+    }
+    if (count > 0) {
+        hipError_t e = hipMalloc(&p->d_items, sizeof(qr::RaggedItem) * (size_t)count);
+        if (e != hipSuccess) {
+            delete p;
+            return hip_fail(e, "hipMalloc(plan items)");
+        }
+        e = hipMemcpy(p->d_items, items.data(), sizeof(qr::RaggedItem) * (size_t)count, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(p->d_items);
+            delete p;
+            return hip_fail(e, "hipMemcpy(plan items)");
+        }
+    }
+    *plan = p;
+    return GPUTILS_OK;
+}
+
+int gputils_gels_ragged_plan_destroy(gputils_gels_plan_t plan) {
+    if (plan == nullptr) return GPUTILS_OK;
+    if (plan->d_items) GPUTILS_HIP(hipFree(plan->d_items));
+    delete plan;
+    return GPUTILS_OK;
+}
+
+int gputils_dgels_ragged(gputils_gels_plan_t plan, const double *A, double *B, int refine_steps,
+                         void *stream) {
+    GPUTILS_ARG(plan != nullptr, 1, "plan is NULL");
+    GPUTILS_ARG(refine_steps >= 0 && refine_steps <= 16, 4, "refine_steps must be in [0, 16]");
+    if (plan->count == 0) return GPUTILS_OK;
+    GPUTILS_ARG(A != nullptr, 2, "A is NULL");
+    GPUTILS_ARG(B != nullptr, 3, "B is NULL");
+    hipStream_t st = as_stream(stream);
+    for (const auto &bk : plan->buckets) {
+        const qr::RaggedItem *it = plan->d_items + bk.begin;
+        if (bk.maxe == 2048)
+            hipLaunchKernelGGL((qr::gels_ragged_kernel<2048>), dim3(bk.grid), dim3(qr::kThreads), 0, st, it, bk.count, A, B, refine_steps);
+        else if (bk.maxe == 8192)
+            hipLaunchKernelGGL((qr::gels_ragged_kernel<8192>), dim3(bk.grid), dim3(qr::kThreads), 0, st, it, bk.count, A, B, refine_steps);
+        else
+            hipLaunchKernelGGL((qr::gels_ragged_kernel<32768>), dim3(bk.grid), dim3(qr::kThreads), 0, st, it, bk.count, A, B, refine_steps);
+        int rc = launch_check("gels_ragged_kernel");
+        if (rc) return rc;
+    }
+    return GPUTILS_OK;
+}
+
+}  // extern "C"

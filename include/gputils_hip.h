@@ -159,6 +159,55 @@ int gputils_dposv_mixed_batched(int64_t n, int64_t nrhs, const double *A, int64_
                                 double *X, int64_t ldx, int64_t strideX, double tol, int max_iter,
                                 int *iters, int *info, int64_t batch, void *stream);
 
+/* ------------------------------------------------------- QR / least sq --- */
+/* Batched least squares min ||A_i x - b_i|| for m >= n (cublas?gelsBatched
+ * behind DTensor::leastSquaresBatched, tensor.cuh:1340-1394; any nrhs).
+ * A_i is overwritten with its Householder QR factors (LAPACK storage) and
+ * the first n rows of each column of B_i with the solution. */
+int gputils_sgels_batched(int64_t m, int64_t n, int64_t nrhs, float *A, int64_t lda, int64_t strideA,
+                          float *B, int64_t ldb, int64_t strideB, int *info, int64_t batch, void *stream);
+int gputils_dgels_batched(int64_t m, int64_t n, int64_t nrhs, double *A, int64_t lda, int64_t strideA,
+                          double *B, int64_t ldb, int64_t strideB, int *info, int64_t batch, void *stream);
+/* Householder QR, LAPACK storage: R on and above the diagonal, reflector
+ * vectors below, tau_i (n values) at tau + i*strideTau (cusolverDn?geqrf of
+ * QRFactoriser::factorise, tensor.cuh:1866-1889). */
+int gputils_sgeqrf_batched(int64_t m, int64_t n, float *A, int64_t lda, int64_t strideA, float *tau,
+                           int64_t strideTau, int *info, int64_t batch, void *stream);
+int gputils_dgeqrf_batched(int64_t m, int64_t n, double *A, int64_t lda, int64_t strideA, double *tau,
+                           int64_t strideTau, int *info, int64_t batch, void *stream);
+/* C_i <- Q_i^T C_i (trans = 1) or Q_i C_i (trans = 0) with Q_i from the
+ * first k reflectors of a geqrf result (cusolverDn?ormqr side L,
+ * tensor.cuh:1896-1902 and 1946-1952). */
+int gputils_sormqr_batched(int trans, int64_t m, int64_t ncols, int64_t k, const float *A, int64_t lda,
+                           int64_t strideA, const float *tau, int64_t strideTau, float *C, int64_t ldc,
+                           int64_t strideC, int64_t batch, void *stream);
+int gputils_dormqr_batched(int trans, int64_t m, int64_t ncols, int64_t k, const double *A, int64_t lda,
+                           int64_t strideA, const double *tau, int64_t strideTau, double *C, int64_t ldc,
+                           int64_t strideC, int64_t batch, void *stream);
+/* B_i <- R_i^{-1} B_i, R_i upper triangular non-unit n x n (cublas?trsm
+ * left/upper/N/non-unit, tensor.cuh:1903-1907). */
+int gputils_strsm_upper_batched(int64_t n, int64_t nrhs, const float *R, int64_t ldr, int64_t strideR,
+                                float *B, int64_t ldb, int64_t strideB, int64_t batch, void *stream);
+int gputils_dtrsm_upper_batched(int64_t n, int64_t nrhs, const double *R, int64_t ldr, int64_t strideR,
+                                double *B, int64_t ldb, int64_t strideB, int64_t batch, void *stream);
+
+/* Ragged least squares (no reference counterpart: the reference needs
+ * uniform sizes).  A plan holds the per-system sizes and offsets, bucketed
+ * by working-set size and sorted by cost (m n^2, largest first) in device
+ * memory; executing it allocates nothing.  System s is the m[s] x n[s]
+ * column-major FP64 matrix at A + offA[s] (ld = m[s]) and the m[s]-vector at
+ * B + offB[s]; 1 <= n[s] <= m[s] <= 512, n[s] <= 64.  Execution solves every
+ * system with an FP32 Householder QR (FP64 norms and dot products), then
+ * refine_steps FP64 refinements of the augmented system [I A; A^T 0][r; x] =
+ * [b; 0] (Bjorck), which also converge for inconsistent systems; A is read
+ * only and the solution overwrites the first n[s] entries of each b. */
+typedef struct gputils_gels_plan *gputils_gels_plan_t;
+int gputils_gels_ragged_plan_create(gputils_gels_plan_t *plan, int64_t count, const int32_t *m,
+                                    const int32_t *n, const int64_t *offA, const int64_t *offB);
+int gputils_gels_ragged_plan_destroy(gputils_gels_plan_t plan);
+int gputils_dgels_ragged(gputils_gels_plan_t plan, const double *A, double *B, int refine_steps,
+                         void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,167 @@
+"""GPU parity of Householder QR, Q application, triangular solve and least
+squares against the FP64 oracle and the reference suite's golden values
+(test/testTensor.cu X:1012-1039, 1433-1535).
+
+Tolerances: FP64 kernels agree with the oracle to 1e-11 (relative to the
+matrix scale); FP32 to the reference suite's PRECISION_LOW (1e-4)."""
+import numpy as np
+import pytest
+import torch
+
+import golden_values as G
+from gputils_amd import api
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    return (t.to(dtype) if dtype is not None else t).cuda()
+
+
+@pytest.mark.parametrize("dtype,eps", [(torch.float32, G.PRECISION_LOW), (torch.float64, G.PRECISION_HIGH)])
+def test_least_squares_batched_golden(dtype, eps):
+    """X:1012-1039: square 2x2x3 systems, ||A x - b|| < eps."""
+    A = G.cm(G.LS1_A, 2, 2, 3)
+    b = np.array(G.LS1_B).reshape(3, 1, 2)
+    tA, tb = dev(api.from_matrices(torch.from_numpy(A)).numpy(), dtype), dev(b, dtype)
+    api.gels_batched(tA, tb)
+    x = tb.cpu().double().numpy()
+    for k in range(3):
+        assert np.linalg.norm(A[k] @ x[k, 0] - b[k, 0]) < eps * 10
+
+
+@pytest.mark.parametrize("dtype,eps", [(torch.float32, G.PRECISION_LOW), (torch.float64, G.PRECISION_HIGH)])
+def test_qr_least_squares_golden_residual(dtype, eps):
+    """X:1496-1535 via geqrf + ormqr(Q^T) + trsm, as QRFactoriser does."""
+    A = np.array(G.QRLS_A_ROWMAJOR).reshape(1, 4, 3)
+    b = np.array(G.QRLS_B).reshape(1, 1, 4)
+    tA = dev(api.from_matrices(torch.from_numpy(A)).numpy(), dtype)
+    tau = api.geqrf_batched(tA)
+    tb = dev(b, dtype)
+    api.ormqr_batched(tA, tau, tb, trans=True)
+    api.trsm_upper_batched(tA, tb)
+    x = tb.cpu().double().numpy()[0, 0, :3]
+    rel = 1e-6 if dtype == torch.float32 else 1e-12
+    assert abs(np.linalg.norm(A[0] @ x - b[0, 0]) - G.QRLS_RESIDUAL) < max(eps, rel * G.QRLS_RESIDUAL)
+
+
+@pytest.mark.parametrize("m,n", [(4, 3), (20, 3), (7, 7), (64, 64), (200, 48), (600, 30)])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_qr_reconstructs_and_matches_oracle(oracle, m, n, dtype):
+    """X:1433-1490 (QR - A ~ 0) plus element-wise agreement of R, v and tau
+    with the oracle's dgeqr2 (FP64)."""
+    k = 3
+    r = np.random.default_rng(m * 31 + n)
+    A = r.uniform(-100, 100, (k, m, n))
+    if dtype == torch.float32:
+        A = A.astype(np.float32).astype(np.float64)
+    tA = dev(api.from_matrices(torch.from_numpy(A)).numpy(), dtype)
+    tau = api.geqrf_batched(tA)
+    QR = api.to_matrices(tA).cpu().double().numpy()
+    taus = tau.cpu().double().numpy()
+    # Q = H_0..H_{n-1} applied to the identity (m x n), as getQR does
+    I = np.zeros((k, n, m))
+    for b in range(k):
+        I[b, :, :n] = np.eye(n)
+    tQ = dev(I, dtype)
+    api.ormqr_batched(tA, tau, tQ, trans=False)
+    Q = api.to_matrices(tQ).cpu().double().numpy()
+    tol = 1e-4 if dtype == torch.float32 else 1e-11
+    for b in range(k):
+        R = np.triu(QR[b][:n, :n])
+        assert np.linalg.norm(Q[b] @ R - A[b]) <= tol * np.linalg.norm(A[b])
+        assert np.linalg.norm(Q[b].T @ Q[b] - np.eye(n)) <= tol * n
+        if dtype == torch.float64:
+            oqr, otau = oracle.geqrf(A[b])
+            scale = np.abs(oqr).max()
+            assert np.max(np.abs(np.triu(QR[b][:n]) - np.triu(oqr[:n]))) <= 1e-11 * scale
+            assert np.max(np.abs(np.tril(QR[b], -1) - np.tril(oqr, -1))) <= 1e-10
+            assert np.max(np.abs(taus[b] - otau)) <= 1e-12
+
+
+@pytest.mark.parametrize("m,n,nrhs", [(2, 2, 1), (5, 3, 1), (30, 8, 2), (128, 64, 1), (512, 64, 1), (700, 40, 1)])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_gels_matches_oracle(oracle, m, n, nrhs, dtype):
+    k = 4
+    r = np.random.default_rng(m + 7 * n)
+    A = r.standard_normal((k, m, n))
+    B = r.standard_normal((k, nrhs, m))
+    if dtype == torch.float32:
+        A, B = A.astype(np.float32).astype(np.float64), B.astype(np.float32).astype(np.float64)
+    tA = dev(api.from_matrices(torch.from_numpy(A)).numpy(), dtype)
+    tB = dev(B, dtype)
+    info = api.gels_batched(tA, tB)
+    X = tB.cpu().double().numpy()[:, :, :n]
+    assert info.cpu().tolist() == [0] * k
+    for b in range(k):
+        for c in range(nrhs):
+            ref = oracle.gels(A[b], B[b, c]).ravel()
+            cond = np.linalg.cond(A[b])
+            tol = (1e-4 if dtype == torch.float32 else 1e-12) * max(cond, 1.0) * 10
+            assert np.linalg.norm(X[b, c] - ref) <= tol * max(np.linalg.norm(ref), 1.0), (m, n, b, c)
+
+
+def test_trsm_and_ormqr_multi_column(oracle):
+    r = np.random.default_rng(3)
+    k, m, n = 5, 40, 12
+    A = r.standard_normal((k, m, n))
+    C = r.standard_normal((k, 3, m))
+    tA = dev(api.from_matrices(torch.from_numpy(A)).numpy())
+    tau = api.geqrf_batched(tA)
+    tC = dev(C)
+    api.ormqr_batched(tA, tau, tC, trans=True)
+    got = tC.cpu().numpy()
+    for b in range(k):
+        oqr, otau = oracle.geqrf(A[b])
+        ref = oracle.ormqr(oqr, otau, C[b].T, trans=True)
+        assert np.allclose(got[b].T, ref, atol=1e-12)
+
+
+def _ragged_check(oracle, refine_steps, sample=300):
+    from gputils_amd import workloads
+    ms, ns, offA, offB, Af, Bf = workloads.ls_ragged()
+    plan = api.RaggedLeastSquares(ms, ns, offA, offB)
+    tA, tB = dev(Af), dev(Bf)
+    plan(tA, tB, refine_steps=refine_steps)
+    out = tB.cpu().numpy()
+    assert np.array_equal(tA.cpu().numpy(), Af), "A must be read only"
+    rng = np.random.default_rng(0)
+    errs = []
+    for s in rng.choice(len(ms), sample, replace=False):
+        m, n = int(ms[s]), int(ns[s])
+        A = Af[offA[s]:offA[s] + m * n].reshape(n, m).T
+        b = Bf[offB[s]:offB[s] + m]
+        ref = oracle.gels(A, b).ravel()
+        x = out[offB[s]:offB[s] + n]
+        # entries n..m-1 of each right-hand side are left untouched
+        assert np.array_equal(out[offB[s] + n:offB[s] + m], b[n:]), s
+        errs.append((np.linalg.norm(x - ref) / np.linalg.norm(ref), np.linalg.cond(A), m, n, int(s)))
+    return errs
+
+
+def test_ragged_least_squares_config(oracle):
+    """4096 uneven tall inconsistent systems (n in [8,64], m in
+    [max(16,n),512], b random): FP32 QR with FP64 norms, then two FP64
+    augmented-system refinements, vs the FP64 oracle dgels.  The bound is the
+    FP64 forward-error level, 1e-12 * cond(A) (times a safety factor 100)."""
+    errs = _ragged_check(oracle, 2)
+    for err, cond, m, n, s in errs:
+        assert err <= 1e-10 * max(cond, 1.0), (s, m, n, err, cond)
+
+
+def test_ragged_least_squares_unrefined_is_fp32_level(oracle):
+    """Without refinement the result carries the FP32 least-squares error
+    (~u32 * cond^2 for a large residual); this fixes that the refinement, not
+    the factorisation, supplies the FP64 accuracy above."""
+    errs = _ragged_check(oracle, 0, sample=100)
+    worst = max(e / max(c, 1.0) ** 2 for e, c, *_ in errs)
+    assert worst <= 1e-4
+    assert max(e for e, *_ in errs) > 1e-9
+
+
+def test_ragged_plan_rejects_bad_sizes():
+    with pytest.raises(api.GputilsError):
+        api.RaggedLeastSquares([10], [20], [0], [0])   # n > m
+    with pytest.raises(api.GputilsError):
+        api.RaggedLeastSquares([600], [10], [0], [0])  # m > 512
