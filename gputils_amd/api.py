@@ -285,3 +285,64 @@ class RaggedLeastSquares:
                 self._plan = ctypes.c_void_p()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------------ SVD --
+def gesvd_batched(A: torch.Tensor, compute_u: bool = False, sweeps: bool = False):
+    """SVD of every tall m x n matrix of A (tensor layout (k, n, m), m >= n,
+    n <= 64).  Returns (S (k, 1, n), U (k, m, m) or None, VT (k, n, n), info
+    (k,)[, sweeps (k,)]).  A is not modified."""
+    _check_dev(A, "A")
+    k, n, m = A.shape
+    dev, dt = A.device, A.dtype
+    S = torch.empty((k, 1, n), dtype=dt, device=dev)
+    VT = torch.empty((k, n, n), dtype=dt, device=dev)
+    U = torch.empty((k, m, m), dtype=dt, device=dev) if compute_u else None
+    info = torch.empty(k, dtype=torch.int32, device=dev)
+    sw = torch.empty(k, dtype=torch.int32, device=dev)
+    L = lib()
+    wbytes = L.gputils_gesvd_workspace_bytes(m, n, k)
+    if wbytes < 0:
+        raise GputilsError(f"unsupported SVD shape m={m}, n={n}")
+    work = torch.empty(max(wbytes, 1), dtype=torch.uint8, device=dev) if wbytes > 0 else None
+    check(getattr(L, f"gputils_{_pfx(A)}gesvd_batched")(
+        m, n, _ptr(A), m, m * n, _ptr(S), n, _ptr(U) if U is not None else None, m, m * m, _ptr(VT), n, n * n,
+        _ptr(work) if work is not None else None, _ptr(info), _ptr(sw), k, stream_handle()), "gesvd_batched")
+    return (S, U, VT, info, sw) if sweeps else (S, U, VT, info)
+
+
+def svd_rank_batched(S: torch.Tensor, eps: float = 1e-6) -> torch.Tensor:
+    """rank[i] = #{sigma > eps} for S of shape (k, 1, n); uint32 counts as
+    int32 tensor (k,)."""
+    _check_dev(S, "S")
+    k = S.shape[0]
+    n = S.shape[-1] * S.shape[-2]
+    rank = torch.empty(k, dtype=torch.int32, device=S.device)
+    check(getattr(lib(), f"gputils_{_pfx(S)}svd_rank_batched")(n, _ptr(S), n, eps, _ptr(rank), k,
+                                                               stream_handle()), "svd_rank_batched")
+    return rank
+
+
+def nullspace_batched(A: torch.Tensor, eps: float = 1e-6):
+    """Nullspace of every m x n (m <= n) matrix of A (tensor layout (k, n, m)),
+    as the Nullspace constructor builds it: SVD of A^T with U, rank with
+    eps, N = U[:, rank:] padded with zero columns, P = N N^T.  Returns
+    (N (k, n, n), P (k, n, n), rank (k,))."""
+    _check_dev(A, "A")
+    k, n, m = A.shape
+    if m > n:
+        raise GputilsError("nullspace expects square or fat matrices (m <= n)")
+    At = transpose_batched(A)            # (k, m, n): n x m matrices
+    S, U, _, _ = gesvd_batched(At, compute_u=True)
+    rank = svd_rank_batched(S, eps)
+    N = torch.empty((k, n, n), dtype=A.dtype, device=A.device)
+    P = torch.empty((k, n, n), dtype=A.dtype, device=A.device)
+    check(getattr(lib(), f"gputils_{_pfx(A)}nullspace_build_batched")(
+        n, _ptr(U), n, n * n, _ptr(rank), _ptr(N), n, n * n, _ptr(P), n, n * n, k, stream_handle()),
+        "nullspace_build_batched")
+    return N, P, rank
+
+
+def project(P: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+    """b_i <- P_i b_i in place (Nullspace::project: addAB with B aliasing C)."""
+    return gemm_batched(P, b, b)

@@ -19,128 +19,12 @@
 #include <vector>
 
 #include "common.hpp"
+#include "householder.hpp"
 
 namespace gputils {
 namespace qr {
 
-constexpr int kThreads = 256;
-constexpr int kWaves = kThreads / kWave;
-
-// Order one wavefront's earlier memory writes before its later reads by
-// other lanes (LDS or global): a wavefront-scope fence plus a scheduling
-// barrier.
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
-
-// Reflector of column j (one wavefront).  Scales v in place, writes beta to
-// A[j][j] and returns tau (identical in every lane).
-template <typename T, typename ACC>
-__device__ __forceinline__ T make_reflector(T *A, int64_t lda, int m, int j) {
-    const int lane = threadIdx.x & 63;
-    T *col = A + (int64_t)j * lda;
-    ACC s = ACC(0);
-    for (int i = j + 1 + lane; i < m; i += kWave) {
-        const ACC x = (ACC)col[i];
-        s += x * x;
-    }
-    s = wave_sum(s);
-    const ACC alpha = (ACC)col[j];
-    if (s == ACC(0)) return T(0);  // H = I (dlarfg with x = 0)
-    const ACC norm = sqrt(alpha * alpha + s);
-    const ACC beta = (alpha >= ACC(0)) ? -norm : norm;
-    const ACC tau = (beta - alpha) / beta;
-    const T scale = (T)(ACC(1) / (alpha - beta));
-    for (int i = j + 1 + lane; i < m; i += kWave) col[i] *= scale;
-    wave_sync();
-    if (lane == 0) col[j] = (T)beta;
-    return (T)tau;
-}
-
-// x <- (I - tau v v^T) x on rows j..m-1 of column c, v = [1; A[j+1:m, j]]
-// (one wavefront).
-template <typename T, typename ACC>
-__device__ __forceinline__ void apply_reflector_col(T *A, int64_t lda, int m, int j, T tau, int c) {
-    const int lane = threadIdx.x & 63;
-    const T *v = A + (int64_t)j * lda;
-    T *x = A + (int64_t)c * lda;
-    ACC w = ACC(0);
-    for (int i = j + 1 + lane; i < m; i += kWave) w += (ACC)v[i] * (ACC)x[i];
-    w = wave_sum(w) + (ACC)x[j];
-    const T tw = (T)((ACC)tau * w);
-    for (int i = j + 1 + lane; i < m; i += kWave) x[i] -= tw * v[i];
-    wave_sync();
-    if (lane == 0) x[j] -= tw;
-}
-
-// Householder QR of the m x n (m >= n) matrix at A (LDS or global), tau[j]
-// for j < n.  s_tau: two LDS slots.  Ends with a barrier.
-template <typename T, typename ACC>
-__device__ __forceinline__ void hqr_factor(T *A, int64_t lda, int m, int n, T *tau, T *s_tau) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (n <= 0) return;
-    if (wave == 0) {
-        const T t = make_reflector<T, ACC>(A, lda, m, 0);
-        if (lane == 0) {
-            s_tau[0] = t;
-            tau[0] = t;
-        }
-    }
-    __syncthreads();
-    for (int j = 0; j < n; ++j) {
-        const T t = s_tau[j & 1];
-        if (j + 1 < n && wave == ((j + 1) & (kWaves - 1))) {
-            apply_reflector_col<T, ACC>(A, lda, m, j, t, j + 1);
-            const T t1 = make_reflector<T, ACC>(A, lda, m, j + 1);
-            if (lane == 0) {
-                s_tau[(j + 1) & 1] = t1;
-                tau[j + 1] = t1;
-            }
-        }
-        if (t != T(0)) {
-            for (int c = j + 2 + ((wave - (j + 2)) & (kWaves - 1)); c < n; c += kWaves)
-                apply_reflector_col<T, ACC>(A, lda, m, j, t, c);
-        }
-        __syncthreads();
-    }
-}
-
-// c <- Q^T c (trans) or Q c for one m-vector in LDS, one wavefront; Q from
-// the first k reflectors stored in A / tau.
-template <typename T, typename ACC>
-__device__ __forceinline__ void apply_q_vec(const T *A, int64_t lda, int m, int k, const T *tau, T *c,
-                                            bool trans) {
-    const int lane = threadIdx.x & 63;
-    for (int t = 0; t < k; ++t) {
-        const int j = trans ? t : (k - 1 - t);
-        const T tj = tau[j];
-        if (tj == T(0)) continue;
-        const T *v = A + (int64_t)j * lda;
-        ACC w = ACC(0);
-        for (int i = j + 1 + lane; i < m; i += kWave) w += (ACC)v[i] * (ACC)c[i];
-        w = wave_sum(w) + (ACC)c[j];
-        const T tw = (T)((ACC)tj * w);
-        for (int i = j + 1 + lane; i < m; i += kWave) c[i] -= tw * v[i];
-        wave_sync();
-        if (lane == 0) c[j] -= tw;
-        wave_sync();
-    }
-}
-
-// Solve R y = c in place (R upper triangular n x n at A, c in LDS), one
-// wavefront, column-oriented back substitution.
-template <typename T>
-__device__ __forceinline__ void trsv_upper(const T *A, int64_t lda, int n, T *c) {
-    const int lane = threadIdx.x & 63;
-    for (int k = n - 1; k >= 0; --k) {
-        const T ck = c[k] / A[k + (int64_t)k * lda];
-        for (int i = lane; i < k; i += kWave) c[i] -= A[i + (int64_t)k * lda] * ck;
-        wave_sync();
-        if (lane == 0) c[k] = ck;
-        wave_sync();
-    }
-}
+using namespace hh;
 
 template <typename T>
 struct Acc;

@@ -1,0 +1,530 @@
+// svd.hip -- batched SVD, rank count and nullspace construction for gfx950.
+//
+// Replaces the host loop of cusolverDn?gesvd calls behind Svd<T>
+// (include/tensor.cuh:1624-1676, jobu 'A' or 'N', jobvt 'A'), the rank
+// kernel k_countNonzeroSingularValues (tensor.cuh:1484-1491, 1600-1609) and
+// the per-matrix slice/copy/GEMM loop of the Nullspace constructor
+// (tensor.cuh:2046-2079).
+//
+// One 256-thread workgroup per matrix, FP64 arithmetic for both float and
+// double data:
+//   1. G = A (m x n, m >= n) is staged in LDS and factorised G = Q [R; 0]
+//      (Householder, LAPACK conventions).  LAPACK dgesvd takes the same
+//      QR-first path for m >= 1.6 n, so Q[:, n:m] is the completion of U it
+//      returns there.
+//   2. One-sided (Hestenes) Jacobi on the columns of W = R, V accumulated.
+//      Parallel tournament ordering: n/2 disjoint column pairs per round,
+//      8 lanes per pair, one barrier per round; sweeps repeat until no pair
+//      needs a rotation, |w_p . w_q| <= n eps ||w_p|| ||w_q||.
+//   3. sigma_j = ||w_j||, sorted descending; for nonzero sigma_j, v_j is
+//      signed so that its largest-magnitude entry is negative and
+//      u_j = w_j / sigma_j.  For a numerically zero sigma_j
+//      (<= m eps sigma_max) v_j keeps its sign and u_j is the first unit
+//      vector e_k whose residual against the columns already placed exceeds
+//      1/2 (two Gram-Schmidt passes), as LAPACK's bidiagonal QR leaves it.
+//   4. U = Q diag(U_R, I_{m-n}), formed up to 16 columns at a time with one
+//      16-lane group per column.
+#include <algorithm>
+#include <cfloat>
+
+#include "common.hpp"
+#include "householder.hpp"
+
+namespace gputils {
+namespace svd {
+
+using hh::kThreads;
+using hh::wave_sync;
+
+constexpr int kMaxN = 64;
+constexpr int kMaxSweeps = 40;
+constexpr int kMaxLds = 160 * 1024;
+
+// Dynamic LDS layout, in doubles unless noted.
+struct Layout {
+    int g;      // G (m x n), resident path only
+    int w;      // W (n x n) during Jacobi, then a chunk of U (m x ch)
+    int ws;     // size of the W region
+    int x;      // V (n x n) during Jacobi, then U_R
+    int sig;    // n
+    int tau;    // n + 2 (two lookahead slots)
+    int sgn;    // n
+    int ints;   // pos (n), ord (n), nonzero count (1) as ints
+    int ch;     // U columns per chunk
+    int bytes;  // total
+};
+
+__host__ __device__ inline Layout layout(int m, int n, bool resident) {
+    Layout L;
+    int o = 0;
+    L.g = o;
+    o += resident ? m * n : 0;
+    L.ws = (n * n > 4 * m) ? n * n : 4 * m;
+    L.w = o;
+    o += L.ws;
+    L.x = o;
+    o += n * n;
+    L.sig = o;
+    o += n;
+    L.tau = o;
+    o += n + 2;
+    L.sgn = o;
+    o += n;
+    L.ints = o;
+    o += (2 * n + 2) / 2 + 1;
+    L.ch = L.ws / m < 16 ? L.ws / m : 16;
+    L.bytes = o * 8;
+    return L;
+}
+
+__device__ __forceinline__ double group_sum(double x, int width) {
+    for (int off = 1; off < width; off <<= 1) x += __shfl_xor(x, off, kWave);
+    return x;
+}
+
+// Pair i of round r in a round-robin tournament over n2 (even) players:
+// player 0 is fixed, players 1..n2-1 rotate.  Every unordered pair meets
+// exactly once in the n2 - 1 rounds.
+__device__ __forceinline__ void tournament_pair(int i, int r, int n1, int &p, int &q) {
+    int a, b;
+    if (i == 0) {
+        a = 0;
+        b = 1 + r;
+    } else {
+        a = 1 + (r + i) % n1;
+        b = 1 + (r - i + n1) % n1;
+    }
+    p = a < b ? a : b;
+    q = a < b ? b : a;
+}
+
+template <typename T, bool RESIDENT>
+__global__ __launch_bounds__(kThreads) void gesvd_kernel(int m, int n, const T *__restrict__ A, int64_t lda,
+                                                         int64_t strideA, T *S, int64_t strideS, T *U,
+                                                         int64_t ldu, int64_t strideU, T *VT, int64_t ldvt,
+                                                         int64_t strideVT, double *work, int *info,
+                                                         int *sweeps_out, int64_t batch) {
+    extern __shared__ double lds[];
+    const int64_t b = blockIdx.x;
+    if (b >= batch) return;
+    const Layout L = layout(m, n, RESIDENT);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double *G = RESIDENT ? lds + L.g : work + b * (int64_t)m * n;
+    double *W = lds + L.w;
+    double *X = lds + L.x;
+    double *sig = lds + L.sig;
+    double *tau = lds + L.tau;
+    double *s_tau = tau + n;
+    double *sgnv = lds + L.sgn;
+    int *pos = reinterpret_cast<int *>(lds + L.ints);
+    int *ord = pos + n;
+    int *s_nz = ord + n;
+
+    // 1. stage and factorise
+    const T *a = A + b * strideA;
+    for (int e = tid; e < m * n; e += kThreads) {
+        const int i = e % m, j = e / m;
+        G[i + (int64_t)j * m] = (double)a[i + (int64_t)j * lda];
+    }
+    __syncthreads();
+    hh::hqr_factor<double, double>(G, m, m, n, tau, s_tau);
+    for (int e = tid; e < n * n; e += kThreads) {
+        const int i = e % n, j = e / n;
+        W[e] = (i <= j) ? G[i + (int64_t)j * m] : 0.0;
+        X[e] = (i == j) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+
+    // 2. one-sided Jacobi on W, V accumulated in X
+    const int n2 = n + (n & 1), n1 = n2 - 1, npairs = n2 >> 1;
+    const int grp = tid >> 3, gl = tid & 7;
+    const double tol = (double)(n > 2 ? n : 2) * DBL_EPSILON;
+    int sweeps = 0;
+    bool converged = (n1 == 0);
+    while (!converged && sweeps < kMaxSweeps) {
+        ++sweeps;
+        int rot = 0;
+        for (int r = 0; r < n1; ++r) {
+            if (grp < npairs) {
+                int p, q;
+                tournament_pair(grp, r, n1, p, q);
+                if (q < n) {
+                    double *wp = W + p * n, *wq = W + q * n;
+                    double al = 0.0, be = 0.0, ga = 0.0;
+                    for (int i = gl; i < n; i += 8) {
+                        const double x = wp[i], y = wq[i];
+                        al = fma(x, x, al);
+                        be = fma(y, y, be);
+                        ga = fma(x, y, ga);
+                    }
+                    al = group_sum(al, 8);
+                    be = group_sum(be, 8);
+                    ga = group_sum(ga, 8);
+                    if (fabs(ga) > tol * sqrt(al) * sqrt(be)) {
+                        const double zeta = (be - al) / (2.0 * ga);
+                        const double az = fabs(zeta);
+                        const double t = (az > 1e150) ? 0.5 / zeta
+                                                      : (zeta >= 0.0 ? 1.0 : -1.0) / (az + sqrt(1.0 + zeta * zeta));
+                        const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+                        for (int i = gl; i < n; i += 8) {
+                            const double x = wp[i], y = wq[i];
+                            wp[i] = cs * x - sn * y;
+                            wq[i] = sn * x + cs * y;
+                        }
+                        double *vp = X + p * n, *vq = X + q * n;
+                        for (int i = gl; i < n; i += 8) {
+                            const double x = vp[i], y = vq[i];
+                            vp[i] = cs * x - sn * y;
+                            vq[i] = sn * x + cs * y;
+                        }
+                        rot = 1;
+                    }
+                }
+            }
+            if (r + 1 < n1) __syncthreads();
+        }
+        converged = !__syncthreads_or(rot);
+    }
+    __syncthreads();
+
+    // 3. singular values, order, signs
+    for (int j = grp; j < n; j += kThreads / 8) {
+        double s = 0.0;
+        for (int i = gl; i < n; i += 8) s = fma(W[i + j * n], W[i + j * n], s);
+        s = group_sum(s, 8);
+        if (gl == 0) sig[j] = sqrt(s);
+    }
+    __syncthreads();
+    if (tid < n) {
+        const int j = tid;
+        const double sj = sig[j];
+        double smax = 0.0;
+        int ps = 0;
+        for (int k = 0; k < n; ++k) {
+            const double sk = sig[k];
+            smax = sk > smax ? sk : smax;
+            ps += (sk > sj || (sk == sj && k < j)) ? 1 : 0;
+        }
+        pos[j] = ps;
+        ord[ps] = j;
+        int imax = 0;
+        double vmax = fabs(X[j * n]);
+        for (int i = 1; i < n; ++i) {
+            const double v = fabs(X[i + j * n]);
+            if (v > vmax) {
+                vmax = v;
+                imax = i;
+            }
+        }
+        // sigma_j <= zt counts as zero: its v_j keeps the accumulated sign
+        // and its u_j comes from the completion below
+        const double zt = smax * (double)m * DBL_EPSILON;
+        sgnv[j] = (sj > zt && X[imax + j * n] > 0.0) ? -1.0 : 1.0;
+        if (j == 0) {
+            int nz = 0;
+            for (int k = 0; k < n; ++k) nz += (sig[k] > zt) ? 1 : 0;
+            *s_nz = nz;
+        }
+    }
+    __syncthreads();
+    T *s_out = S + b * strideS;
+    for (int c = tid; c < n; c += kThreads) s_out[c] = (T)sig[ord[c]];
+    T *vt = VT + b * strideVT;
+    for (int e = tid; e < n * n; e += kThreads) {
+        const int rr = e % n, i = e / n, j = ord[rr];
+        vt[rr + (int64_t)i * ldvt] = (T)(sgnv[j] * X[i + j * n]);
+    }
+    if (tid == 0) {
+        info[b] = converged ? 0 : 1;
+        if (sweeps_out) sweeps_out[b] = sweeps;
+    }
+    if (U == nullptr) return;
+    __syncthreads();
+
+    // U_R in output order (X is free once VT is written)
+    const int nz = *s_nz;
+    for (int e = tid; e < n * n; e += kThreads) {
+        const int i = e % n, c = e / n, j = ord[c];
+        X[e] = (c < nz) ? sgnv[j] * W[i + j * n] / sig[j] : 0.0;
+    }
+    __syncthreads();
+    if (wave == 0 && nz < n) {
+        int next = 0;
+        for (int col = nz; col < n; ++col) {
+            double best_x = 0.0, best_norm = -1.0;
+            int best_k = next;
+            for (int k = next; k < n; ++k) {
+                double x = (lane == k) ? 1.0 : 0.0;
+                for (int pass = 0; pass < 2; ++pass) {
+                    for (int c = 0; c < col; ++c) {
+                        const double uc = (lane < n) ? X[lane + c * n] : 0.0;
+                        const double coef = wave_sum(uc * x);
+                        x -= coef * uc;
+                    }
+                }
+                const double nrm = sqrt(wave_sum(x * x));
+                if (nrm > best_norm) {
+                    best_norm = nrm;
+                    best_x = x;
+                    best_k = k;
+                }
+                if (nrm > 0.5) break;
+            }
+            if (lane < n) X[lane + col * n] = best_x / best_norm;
+            wave_sync();
+            if (best_norm > 0.5) next = best_k + 1;
+        }
+    }
+    __syncthreads();
+
+    // 4. U = Q diag(U_R, I): chunk columns in W, one 16-lane group each
+    const int g16 = tid >> 4, l16 = tid & 15;
+    T *u = U + b * strideU;
+    for (int c0 = 0; c0 < m; c0 += L.ch) {
+        const int cc = c0 + g16;
+        if (g16 < L.ch && cc < m) {
+            double *xc = W + g16 * m;
+            for (int i = l16; i < m; i += 16)
+                xc[i] = (cc < n) ? (i < n ? X[i + cc * n] : 0.0) : (i == cc ? 1.0 : 0.0);
+            wave_sync();
+            for (int j = n - 1; j >= 0; --j) {
+                const double t = tau[j];
+                if (t == 0.0) continue;
+                const double *v = G + (int64_t)j * m;
+                double w = 0.0;
+                for (int i = j + 1 + l16; i < m; i += 16) w = fma(v[i], xc[i], w);
+                w = group_sum(w, 16) + xc[j];
+                const double tw = t * w;
+                for (int i = j + 1 + l16; i < m; i += 16) xc[i] -= tw * v[i];
+                wave_sync();
+                if (l16 == 0) xc[j] -= tw;
+                wave_sync();
+            }
+            for (int i = l16; i < m; i += 16) u[i + (int64_t)cc * ldu] = (T)xc[i];
+        }
+    }
+}
+
+template <typename T>
+__global__ void rank_kernel(int n, const T *__restrict__ S, int64_t strideS, T eps, unsigned *rank,
+                            int64_t batch) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= batch) return;
+    unsigned r = 0;
+    for (int i = 0; i < n; ++i) r += (S[b * strideS + i] > eps) ? 1u : 0u;
+    rank[b] = r;
+}
+
+// N_b[:, 0:nul] = U_b[:, r:n] (other columns zero), P_b = N_b N_b^T with
+// nul = n - rank_b.  P is computed on the lower tile triangle and mirrored.
+constexpr int kNbKc = 16;
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void nullspace_build_kernel(int n, const T *__restrict__ U, int64_t ldu,
+                                                                   int64_t strideU, const unsigned *rank, T *N,
+                                                                   int64_t ldn, int64_t strideN, T *P,
+                                                                   int64_t ldp, int64_t strideP, int64_t batch) {
+    extern __shared__ unsigned char dyn_raw[];
+    T *Ns = reinterpret_cast<T *>(dyn_raw);  // kNbKc columns of N, ld n
+    const int64_t b = blockIdx.x;
+    if (b >= batch) return;
+    const int tid = threadIdx.x;
+    const unsigned rk = rank[b];
+    const int r = (int)(rk < (unsigned)n ? rk : (unsigned)n);
+    const int nul = n - r;
+    const T *ub = U + b * strideU;
+    T *nb = N + b * strideN;
+    T *pb = P + b * strideP;
+    for (int e = tid; e < n * n; e += kThreads) {
+        const int i = e % n, c = e / n;
+        nb[i + (int64_t)c * ldn] = (c < nul) ? ub[i + (int64_t)(r + c) * ldu] : T(0);
+    }
+    const int nt = (n + 3) / 4;
+    const int ntiles = nt * (nt + 1) / 2;
+    for (int t0 = 0; t0 < ntiles; t0 += kThreads) {
+        const int t = t0 + tid;
+        // tile t -> (ti, tj) with ti >= tj
+        int ti = 0, tj = 0;
+        if (t < ntiles) {
+            ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+            while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+            while (ti * (ti + 1) / 2 > t) --ti;
+            tj = t - ti * (ti + 1) / 2;
+        }
+        T acc[4][4];
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) acc[x][y] = T(0);
+        for (int k0 = 0; k0 < nul; k0 += kNbKc) {
+            const int kc = (nul - k0) < kNbKc ? (nul - k0) : kNbKc;
+            __syncthreads();
+            for (int e = tid; e < kc * n; e += kThreads) {
+                const int i = e % n, c = e / n;
+                Ns[e] = ub[i + (int64_t)(r + k0 + c) * ldu];
+            }
+            __syncthreads();
+            if (t < ntiles) {
+                for (int k = 0; k < kc; ++k) {
+                    T ri[4], rj[4];
+#pragma unroll
+                    for (int x = 0; x < 4; ++x) {
+                        const int i = 4 * ti + x, j = 4 * tj + x;
+                        ri[x] = (i < n) ? Ns[i + k * n] : T(0);
+                        rj[x] = (j < n) ? Ns[j + k * n] : T(0);
+                    }
+#pragma unroll
+                    for (int x = 0; x < 4; ++x)
+#pragma unroll
+                        for (int y = 0; y < 4; ++y) acc[x][y] = fma(ri[x], rj[y], acc[x][y]);
+                }
+            }
+        }
+        if (t < ntiles) {
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+                for (int y = 0; y < 4; ++y) {
+                    const int i = 4 * ti + x, j = 4 * tj + y;
+                    if (i < n && j < n) {
+                        pb[i + (int64_t)j * ldp] = acc[x][y];
+                        pb[j + (int64_t)i * ldp] = acc[x][y];
+                    }
+                }
+        }
+    }
+}
+
+inline bool resident_fits(int m, int n) { return layout(m, n, true).bytes <= kMaxLds; }
+
+template <typename T, bool RES>
+int launch_gesvd(int m, int n, const T *A, int64_t lda, int64_t strideA, T *S, int64_t strideS, T *U,
+                 int64_t ldu, int64_t strideU, T *VT, int64_t ldvt, int64_t strideVT, double *work, int *info,
+                 int *sweeps, int64_t batch, hipStream_t st) {
+    const Layout L = layout(m, n, RES);
+    auto kern = gesvd_kernel<T, RES>;
+    if (L.bytes > 64 * 1024)
+        GPUTILS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, L.bytes));
+    hipLaunchKernelGGL(kern, dim3((unsigned)batch), dim3(kThreads), L.bytes, st, m, n, A, lda, strideA, S,
+                       strideS, U, ldu, strideU, VT, ldvt, strideVT, work, info, sweeps, batch);
+    return launch_check("gesvd_kernel");
+}
+
+template <typename T>
+int gesvd_batched(int m, int n, const T *A, int64_t lda, int64_t strideA, T *S, int64_t strideS, T *U,
+                  int64_t ldu, int64_t strideU, T *VT, int64_t ldvt, int64_t strideVT, void *work, int *info,
+                  int *sweeps, int64_t batch, void *stream) {
+    GPUTILS_ARG(n >= 1 && n <= kMaxN, 2, "n must be in [1, 64]");
+    GPUTILS_ARG(m >= n, 1, "m must be >= n (tall matrices; transpose a fat one first)");
+    GPUTILS_ARG(lda >= m, 4, "lda < m");
+    GPUTILS_ARG(strideA >= 0, 5, "negative strideA");
+    GPUTILS_ARG(strideS >= 0, 7, "negative strideS");
+    GPUTILS_ARG(U == nullptr || ldu >= m, 9, "ldu < m");
+    GPUTILS_ARG(strideU >= 0, 10, "negative strideU");
+    GPUTILS_ARG(ldvt >= n, 12, "ldvt < n");
+    GPUTILS_ARG(strideVT >= 0, 13, "negative strideVT");
+    GPUTILS_ARG(info != nullptr, 15, "info is NULL");
+    GPUTILS_ARG(batch >= 0 && batch <= 0x7fffffff, 17, "batch out of range");
+    if (batch == 0) return 0;
+    GPUTILS_ARG(A != nullptr, 3, "A is NULL");
+    GPUTILS_ARG(S != nullptr, 6, "S is NULL");
+    GPUTILS_ARG(VT != nullptr, 11, "VT is NULL");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (resident_fits(m, n))
+        return launch_gesvd<T, true>(m, n, A, lda, strideA, S, strideS, U, ldu, strideU, VT, ldvt, strideVT,
+                                     nullptr, info, sweeps, batch, st);
+    GPUTILS_ARG(layout(m, n, false).bytes <= kMaxLds, 1, "m too large for the batched SVD");
+    GPUTILS_ARG(work != nullptr, 14, "work is NULL (query gputils_gesvd_workspace_bytes)");
+    return launch_gesvd<T, false>(m, n, A, lda, strideA, S, strideS, U, ldu, strideU, VT, ldvt, strideVT,
+                                  static_cast<double *>(work), info, sweeps, batch, st);
+}
+
+template <typename T>
+int rank_batched(int n, const T *S, int64_t strideS, T eps, unsigned *rank, int64_t batch, void *stream) {
+    GPUTILS_ARG(n >= 0, 1, "negative n");
+    GPUTILS_ARG(strideS >= 0, 3, "negative strideS");
+    GPUTILS_ARG(batch >= 0, 6, "negative batch");
+    if (batch == 0) return 0;
+    GPUTILS_ARG(S != nullptr, 2, "S is NULL");
+    GPUTILS_ARG(rank != nullptr, 5, "rank is NULL");
+    const unsigned blocks = (unsigned)((batch + 255) / 256);
+    hipLaunchKernelGGL(rank_kernel<T>, dim3(blocks), dim3(256), 0, static_cast<hipStream_t>(stream), n, S, strideS,
+                       eps, rank, batch);
+    return launch_check("rank_kernel");
+}
+
+template <typename T>
+int nullspace_build(int n, const T *U, int64_t ldu, int64_t strideU, const unsigned *rank, T *N, int64_t ldn,
+                    int64_t strideN, T *P, int64_t ldp, int64_t strideP, int64_t batch, void *stream) {
+    GPUTILS_ARG(n >= 1, 1, "n must be >= 1");
+    GPUTILS_ARG(ldu >= n, 3, "ldu < n");
+    GPUTILS_ARG(strideU >= 0, 4, "negative strideU");
+    GPUTILS_ARG(ldn >= n, 7, "ldn < n");
+    GPUTILS_ARG(strideN >= 0, 8, "negative strideN");
+    GPUTILS_ARG(ldp >= n, 10, "ldp < n");
+    GPUTILS_ARG(strideP >= 0, 11, "negative strideP");
+    GPUTILS_ARG(batch >= 0 && batch <= 0x7fffffff, 12, "batch out of range");
+    if (batch == 0) return 0;
+    GPUTILS_ARG(U != nullptr, 2, "U is NULL");
+    GPUTILS_ARG(rank != nullptr, 5, "rank is NULL");
+    GPUTILS_ARG(N != nullptr, 6, "N is NULL");
+    GPUTILS_ARG(P != nullptr, 9, "P is NULL");
+    const size_t lds = (size_t)kNbKc * n * sizeof(T);
+    GPUTILS_ARG(lds <= (size_t)kMaxLds, 1, "n too large");
+    auto kern = nullspace_build_kernel<T>;
+    if (lds > 64 * 1024)
+        GPUTILS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kern, dim3((unsigned)batch), dim3(kThreads), lds, static_cast<hipStream_t>(stream), n, U, ldu,
+                       strideU, rank, N, ldn, strideN, P, ldp, strideP, batch);
+    return launch_check("nullspace_build_kernel");
+}
+
+}  // namespace svd
+}  // namespace gputils
+
+using namespace gputils;
+
+extern "C" {
+
+int64_t gputils_gesvd_workspace_bytes(int m, int n, int64_t batch) {
+    if (m < n || n < 1 || n > svd::kMaxN || batch < 0) return -1;
+    if (svd::resident_fits(m, n)) return 0;
+    return (int64_t)m * n * (int64_t)sizeof(double) * batch;
+}
+
+int gputils_sgesvd_batched(int m, int n, const float *A, int64_t lda, int64_t strideA, float *S, int64_t strideS,
+                           float *U, int64_t ldu, int64_t strideU, float *VT, int64_t ldvt, int64_t strideVT,
+                           void *work, int *info, int *sweeps, int64_t batch, void *stream) {
+    return svd::gesvd_batched<float>(m, n, A, lda, strideA, S, strideS, U, ldu, strideU, VT, ldvt, strideVT, work,
+                                     info, sweeps, batch, stream);
+}
+int gputils_dgesvd_batched(int m, int n, const double *A, int64_t lda, int64_t strideA, double *S, int64_t strideS,
+                           double *U, int64_t ldu, int64_t strideU, double *VT, int64_t ldvt, int64_t strideVT,
+                           void *work, int *info, int *sweeps, int64_t batch, void *stream) {
+    return svd::gesvd_batched<double>(m, n, A, lda, strideA, S, strideS, U, ldu, strideU, VT, ldvt, strideVT, work,
+                                      info, sweeps, batch, stream);
+}
+
+int gputils_ssvd_rank_batched(int n, const float *S, int64_t strideS, float eps, unsigned *rank, int64_t batch,
+                              void *stream) {
+    return svd::rank_batched<float>(n, S, strideS, eps, rank, batch, stream);
+}
+int gputils_dsvd_rank_batched(int n, const double *S, int64_t strideS, double eps, unsigned *rank, int64_t batch,
+                              void *stream) {
+    return svd::rank_batched<double>(n, S, strideS, eps, rank, batch, stream);
+}
+
+int gputils_snullspace_build_batched(int n, const float *U, int64_t ldu, int64_t strideU, const unsigned *rank,
+                                     float *N, int64_t ldn, int64_t strideN, float *P, int64_t ldp, int64_t strideP,
+                                     int64_t batch, void *stream) {
+    return svd::nullspace_build<float>(n, U, ldu, strideU, rank, N, ldn, strideN, P, ldp, strideP, batch, stream);
+}
+int gputils_dnullspace_build_batched(int n, const double *U, int64_t ldu, int64_t strideU, const unsigned *rank,
+                                     double *N, int64_t ldn, int64_t strideN, double *P, int64_t ldp,
+                                     int64_t strideP, int64_t batch, void *stream) {
+    return svd::nullspace_build<double>(n, U, ldu, strideU, rank, N, ldn, strideN, P, ldp, strideP, batch, stream);
+}
+
+}  // extern "C"

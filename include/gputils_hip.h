@@ -208,6 +208,43 @@ int gputils_gels_ragged_plan_destroy(gputils_gels_plan_t plan);
 int gputils_dgels_ragged(gputils_gels_plan_t plan, const double *A, double *B, int refine_steps,
                          void *stream);
 
+/* ------------------------------------------------------------ SVD / rank --- */
+/* A_i = U_i diag(S_i) VT_i for tall m x n matrices (m >= n, 1 <= n <= 64):
+ * replaces the per-matrix cusolverDn?gesvd loop of Svd::factorise
+ * (tensor.cuh:1624-1676, jobu 'A' when U != NULL else 'N', jobvt 'A').
+ * S_i: n values, descending.  VT_i: n x n (ld ldvt), row j = v_j^T with the
+ * largest-magnitude entry of v_j negative.  U_i: m x m (ld ldu) or NULL.
+ * A is read only.  FP64 arithmetic inside for both types.  info[i] = 0, or
+ * 1 if the Jacobi sweeps did not converge; sweeps (may be NULL) receives the
+ * sweep count.  work: gputils_gesvd_workspace_bytes(m, n, batch) bytes of
+ * device memory (0 bytes, work may be NULL, when the matrix fits in LDS). */
+int64_t gputils_gesvd_workspace_bytes(int m, int n, int64_t batch);
+int gputils_sgesvd_batched(int m, int n, const float *A, int64_t lda, int64_t strideA, float *S, int64_t strideS,
+                           float *U, int64_t ldu, int64_t strideU, float *VT, int64_t ldvt, int64_t strideVT,
+                           void *work, int *info, int *sweeps, int64_t batch, void *stream);
+int gputils_dgesvd_batched(int m, int n, const double *A, int64_t lda, int64_t strideA, double *S, int64_t strideS,
+                           double *U, int64_t ldu, int64_t strideU, double *VT, int64_t ldvt, int64_t strideVT,
+                           void *work, int *info, int *sweeps, int64_t batch, void *stream);
+
+/* rank[i] = #{j < n : S_i[j] > eps} (Svd::rank and k_countNonzeroSingularValues,
+ * tensor.cuh:1484-1491, 1600-1609).  Unlike the reference, the count is
+ * written, not accumulated, so repeated calls give the same result. */
+int gputils_ssvd_rank_batched(int n, const float *S, int64_t strideS, float eps, unsigned *rank, int64_t batch,
+                              void *stream);
+int gputils_dsvd_rank_batched(int n, const double *S, int64_t strideS, double eps, unsigned *rank, int64_t batch,
+                              void *stream);
+
+/* Nullspace construction from the left singular vectors of A_i^T
+ * (Nullspace ctor loop, tensor.cuh:2058-2078): with nul = n - rank[i],
+ * N_i[:, 0:nul] = U_i[:, rank[i]:n], remaining columns zero, and
+ * P_i = N_i N_i^T; all matrices n x n. */
+int gputils_snullspace_build_batched(int n, const float *U, int64_t ldu, int64_t strideU, const unsigned *rank,
+                                     float *N, int64_t ldn, int64_t strideN, float *P, int64_t ldp, int64_t strideP,
+                                     int64_t batch, void *stream);
+int gputils_dnullspace_build_batched(int n, const double *U, int64_t ldu, int64_t strideU, const unsigned *rank,
+                                     double *N, int64_t ldn, int64_t strideN, double *P, int64_t ldp,
+                                     int64_t strideP, int64_t batch, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -252,11 +252,13 @@ int orc_dgesvd(int64_t m, int64_t n, double *A, int64_t lda, double *S,
     for (int64_t jj = 0; jj < n; ++jj) {
         int64_t j = ord[jj];
         S[jj] = sig[j];
-        /* sign convention: largest-magnitude entry of v_j is negative */
+        /* sign convention: largest-magnitude entry of v_j is negative for a
+         * nonzero sigma_j; vectors of zero sigma keep their accumulated sign
+         * (an untouched identity column stays e_j, as in LAPACK) */
         int64_t imax = 0;
         for (int64_t i = 1; i < n; ++i)
             if (fabs(V[IDX(i, j, n)]) > fabs(V[IDX(imax, j, n)])) imax = i;
-        double sgn = (V[IDX(imax, j, n)] > 0.0) ? -1.0 : 1.0;
+        double sgn = (sig[j] > thresh && V[IDX(imax, j, n)] > 0.0) ? -1.0 : 1.0;
         for (int64_t i = 0; i < n; ++i) VT[IDX(jj, i, ldvt)] = sgn * V[IDX(i, j, n)];
         if (U && sig[j] > thresh) {
             for (int64_t i = 0; i < m; ++i) U[IDX(i, jj, ldu)] = sgn * A[IDX(i, j, lda)] / sig[j];

@@ -1,0 +1,247 @@
+"""GPU parity of the batched SVD, rank count, nullspace construction and
+projection against the oracle and the reference suite's golden values
+(test/testTensor.cu X:1058-1193 for SVD/rank, X:1552-1657 for Nullspace).
+
+Singular values agree with the FP64 oracle to 1e-12 relative to sigma_max
+(FP64 data; float data to 2e-6).  Singular vectors of distinct nonzero
+singular values agree element-wise, signs included (both sides put the
+largest-magnitude entry of v_j negative).  Vectors of zero or repeated
+singular values are compared as subspaces (projectors), as the reference
+pins them only up to basis."""
+import numpy as np
+import pytest
+import torch
+
+import golden_values as G
+from gputils_amd import api
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    return (t.to(dtype) if dtype is not None else t).cuda()
+
+
+def tensor_of(mats, dtype):
+    """(k, m, n) matrices -> device tensor in (k, n, m) storage."""
+    return dev(api.from_matrices(torch.from_numpy(np.ascontiguousarray(mats))).numpy(), dtype)
+
+
+def svd(mats, dtype, compute_u=True):
+    S, U, VT, info, sw = api.gesvd_batched(tensor_of(mats, dtype), compute_u=compute_u, sweeps=True)
+    S = S.cpu().double().numpy()[:, 0, :]
+    VT = api.to_matrices(VT).cpu().double().numpy()
+    U = api.to_matrices(U).cpu().double().numpy() if U is not None else None
+    return S, U, VT, info.cpu().numpy(), sw.cpu().numpy()
+
+
+@pytest.mark.parametrize("dtype,eps", [(torch.float32, G.PRECISION_LOW), (torch.float64, G.PRECISION_HIGH)])
+def test_singular_values_golden(dtype, eps):
+    """X:1058-1078: sigma_0, sigma_1 of an 8 x 3 matrix (MATLAB values)."""
+    S, U, VT, info, _ = svd(G.cm(G.SVD8_B, 8, 3), dtype)
+    assert info.tolist() == [0]
+    assert abs(S[0, 0] - G.SVD8_S01[0]) < eps and abs(S[0, 1] - G.SVD8_S01[1]) < eps
+    assert U is not None and U.shape == (1, 8, 8)
+
+
+@pytest.mark.parametrize("dtype,eps", [(torch.float32, 10 * G.PRECISION_LOW), (torch.float64, G.PRECISION_HIGH)])
+def test_singular_vectors_multiple_matrices_golden(dtype, eps):
+    """X:1117-1166: three 3 x 2 matrices (ranks 2, 1, 1): all six sigma, the
+    first four entries of V' and all 27 entries of U, element-wise."""
+    A = G.cm(G.SVD3_A, 3, 2, 3)
+    S, U, VT, info, _ = svd(A, dtype)
+    assert np.allclose(S.ravel(), G.SVD3_S, atol=eps)
+    vt_flat = np.concatenate([VT[b].ravel(order="F") for b in range(3)])
+    assert np.allclose(vt_flat[:4], G.SVD3_V4, atol=eps)
+    u_flat = np.concatenate([U[b].ravel(order="F") for b in range(3)])
+    assert np.allclose(u_flat, G.SVD3_U, atol=eps), np.round(u_flat - np.array(G.SVD3_U), 6)
+
+
+@pytest.mark.parametrize("dtype,eps", [(torch.float32, G.PRECISION_LOW), (torch.float64, G.PRECISION_HIGH)])
+def test_rank_golden(dtype, eps):
+    """X:1175-1193: ranks 3, 2, 1 of three 4 x 3 matrices."""
+    A = tensor_of(G.cm(G.RANK_A, 4, 3, 3), dtype)
+    S, _, _, _ = api.gesvd_batched(A)
+    r = api.svd_rank_batched(S, eps)
+    assert r.cpu().tolist() == G.RANK_EXPECTED
+    # the count is written, not accumulated: a second call gives the same
+    assert api.svd_rank_batched(S, eps).cpu().tolist() == G.RANK_EXPECTED
+
+
+SHAPES = [(1, 1), (5, 1), (3, 2), (10, 4), (17, 9), (30, 30), (64, 64), (96, 48), (200, 50), (300, 64),
+          (1000, 20)]
+
+
+@pytest.mark.parametrize("m,n", SHAPES)
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_svd_matches_oracle(oracle, m, n, dtype):
+    k = 3
+    r = np.random.default_rng(1000 * m + n)
+    A = r.standard_normal((k, m, n))
+    if dtype == torch.float32:
+        A = A.astype(np.float32).astype(np.float64)
+    S, U, VT, info, sw = svd(A, dtype)
+    assert info.tolist() == [0] * k
+    tol = 1e-12 if dtype == torch.float64 else 2e-6
+    for b in range(k):
+        So, Uo, VTo, _ = oracle.gesvd(A[b].copy())
+        smax = So[0]
+        assert np.max(np.abs(S[b] - So)) <= tol * smax * max(1, n / 8)
+        # distinct, well separated sigma: vectors agree element-wise
+        gap = np.min(np.abs(np.diff(So))) / smax if n > 1 else 1.0
+        vtol = tol * 100 / max(gap, 1e-3)
+        assert np.max(np.abs(VT[b] - VTo)) <= vtol, (b, np.max(np.abs(VT[b] - VTo)))
+        assert np.max(np.abs(U[b][:, :n] - Uo[:, :n])) <= vtol
+        # U orthogonal, completion spans the complement, reconstruction
+        assert np.allclose(U[b].T @ U[b], np.eye(m), atol=tol * 100)
+        Pc, Po = U[b][:, n:] @ U[b][:, n:].T, Uo[:, n:] @ Uo[:, n:].T
+        assert np.allclose(Pc, Po, atol=tol * 100)
+        assert np.linalg.norm(U[b][:, :n] * S[b] @ VT[b] - A[b]) <= tol * 10 * smax * np.sqrt(m * n)
+    assert sw.max() <= 15
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_svd_without_u_and_read_only_input(dtype):
+    r = np.random.default_rng(5)
+    A = r.standard_normal((4, 40, 12))
+    tA = tensor_of(A, dtype)
+    before = tA.clone()
+    S1, U1, VT1, _ = api.gesvd_batched(tA, compute_u=False)
+    S2, U2, VT2, _ = api.gesvd_batched(tA, compute_u=True)
+    assert U1 is None
+    assert torch.equal(tA, before)
+    assert torch.equal(S1, S2) and torch.equal(VT1, VT2)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_svd_rank_deficient_and_zero(oracle, dtype):
+    """Exact low-rank constructions, a zero matrix and zero columns: the
+    trailing sigma vanish, rank is exact, U and V stay orthogonal and the
+    null directions agree with the oracle as subspaces."""
+    r = np.random.default_rng(11)
+    mats = []
+    for (m, n, rk) in [(20, 8, 3)] * 3:
+        mats.append(r.standard_normal((m, rk)) @ r.standard_normal((rk, n)))
+    mats = np.stack(mats)
+    mats[1] = 0.0
+    mats[2][:, [1, 4]] = 0.0
+    S, U, VT, info, _ = svd(mats, dtype)
+    tol = 1e-12 if dtype == torch.float64 else 1e-5
+    ranks = api.svd_rank_batched(dev(S[:, None, :], dtype), 1e-6 if dtype == torch.float64 else 1e-3)
+    for b in range(3):
+        So, Uo, VTo, _ = oracle.gesvd(mats[b].copy())
+        smax = max(So[0], 1.0)
+        assert np.max(np.abs(S[b] - So)) <= tol * smax * 10
+        assert np.allclose(U[b].T @ U[b], np.eye(20), atol=tol * 10)
+        assert np.allclose(VT[b] @ VT[b].T, np.eye(8), atol=tol * 10)
+        rk = int(np.sum(So > 1e-6 * smax))
+        assert ranks[b].item() == rk
+        assert np.allclose(U[b][:, :rk] @ U[b][:, :rk].T, Uo[:, :rk] @ Uo[:, :rk].T, atol=tol * 100)
+        assert np.allclose(VT[b][:rk].T @ VT[b][:rk], VTo[:rk].T @ VTo[:rk], atol=tol * 100)
+    # the zero matrix: U = I and V' = I, as LAPACK leaves them
+    assert np.array_equal(S[1], np.zeros(8))
+    assert np.allclose(U[1], np.eye(20)) and np.allclose(VT[1], np.eye(8))
+
+
+def test_svd_argument_checks():
+    A = torch.zeros((2, 3, 5), dtype=torch.float64, device="cuda")   # 5 x 3: fine
+    api.gesvd_batched(A)
+    with pytest.raises(api.GputilsError):
+        api.gesvd_batched(torch.zeros((2, 5, 3), dtype=torch.float64, device="cuda"))  # fat
+    with pytest.raises(api.GputilsError):
+        api.gesvd_batched(torch.zeros((1, 65, 80), dtype=torch.float64, device="cuda"))  # n > 64
+
+
+# ------------------------------------------------------------ nullspace --
+@pytest.mark.parametrize("dtype,eps", [(torch.float32, G.PRECISION_LOW), (torch.float64, G.PRECISION_HIGH)])
+def test_nullspace_golden(oracle, dtype, eps):
+    """X:1556-1585: A N = 0 and N^T N = diag(I, 0) for five 3 x 4 matrices,
+    and P equal to the oracle's (basis independent)."""
+    A = G.cm(G.NS_A, 3, 4, 5)
+    N, P, rank = api.nullspace_batched(tensor_of(A, dtype))
+    N = api.to_matrices(N).cpu().double().numpy()
+    P = api.to_matrices(P).cpu().double().numpy()
+    for b in range(5):
+        assert np.linalg.norm(A[b] @ N[b]) < eps
+        NtN = N[b].T @ N[b]
+        nul = 4 - np.linalg.matrix_rank(A[b])
+        assert rank[b].item() == 4 - nul if nul < 4 else rank[b].item() == 0
+        E = np.zeros((4, 4))
+        E[:nul, :nul] = np.eye(nul)
+        assert np.allclose(NtN, E, atol=eps)
+        No, Po, nul_o = oracle.nullspace(A[b])
+        assert nul_o == nul
+        assert np.allclose(P[b], Po, atol=eps)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_nullspace_trivial_is_exactly_zero(dtype):
+    """X:1597-1613: full-rank square matrices give N = 0 exactly."""
+    A = np.array(G.NS_TRIVIAL_ROWMAJOR, dtype=float).reshape(2, 3, 3)
+    N, P, rank = api.nullspace_batched(tensor_of(A, dtype))
+    assert rank.cpu().tolist() == [3, 3]
+    assert torch.count_nonzero(N).item() == 0 and torch.count_nonzero(P).item() == 0
+
+
+@pytest.mark.parametrize("dtype,eps", [(torch.float32, G.PRECISION_LOW), (torch.float64, G.PRECISION_HIGH)])
+def test_project_on_nullspace_golden(dtype, eps):
+    """X:1624-1657: A (P x) = 0 and (N o - P x) . (P x - x) = 0."""
+    A = np.array(G.PROJ_A_ROWMAJOR, dtype=float).reshape(1, 3, 7)
+    N, P, _ = api.nullspace_batched(tensor_of(A, dtype))
+    x = dev(np.array(G.PROJ_X, dtype=float).reshape(1, 1, 7), dtype)
+    proj = api.project(P, x.clone())
+    pn = proj.cpu().double().numpy()[0, 0]
+    assert np.linalg.norm(A[0] @ pn) < eps
+    Nm = api.to_matrices(N).cpu().double().numpy()[0]
+    y = Nm @ np.array(G.PROJ_OTHER, dtype=float)
+    assert (y - pn) @ (pn - np.array(G.PROJ_X, dtype=float)) < eps
+
+
+def test_nullspace_config(oracle):
+    """2048 rank-40 48 x 96 FP32 matrices (sigma in [0.1, 10]): rank 40
+    everywhere, P equal to the oracle's on a sample, A P x ~ 0 for all."""
+    from gputils_amd import workloads
+    A_t, x = workloads.svd_nullspace()
+    tA = dev(A_t)
+    N, P, rank = api.nullspace_batched(tA)
+    assert rank.cpu().tolist() == [40] * 2048
+    Pm = api.to_matrices(P).cpu().double().numpy()
+    Am = api.to_matrices(torch.from_numpy(A_t)).double().numpy()
+    for b in np.random.default_rng(0).choice(2048, 16, replace=False):
+        _, Po, nul = oracle.nullspace(Am[b])
+        assert nul == 56
+        assert np.max(np.abs(Pm[b] - Po)) < 2e-6
+    tb = dev(x)
+    api.project(P, tb)
+    z = tb.cpu().double().numpy()[:, 0, :]
+    res = np.einsum("kij,kj->ki", Am, z)
+    assert np.max(np.linalg.norm(res, axis=1) / np.linalg.norm(x[:, 0, :], axis=1)) < 1e-5
+
+
+def test_project_graph_replays_match_eager():
+    """Nullspace::project captured in a HIP graph and replayed repeatedly,
+    with eager work in between: every replay equals the eager result."""
+    r = np.random.default_rng(2)
+    k, n = 64, 96
+    Q = np.linalg.qr(r.standard_normal((k, n, n)))[0][:, :, :30]
+    P = dev(api.from_matrices(torch.from_numpy(Q @ np.swapaxes(Q, 1, 2))).numpy(), torch.float32)
+    b0 = dev(r.standard_normal((k, 1, n)), torch.float32)
+    eager = api.project(P, b0.clone())
+    b = b0.clone()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        api.project(P, b)            # warm-up outside capture
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g, stream=s):
+            api.project(P, b)
+    torch.cuda.synchronize()
+    for rep in range(4):
+        b.copy_(b0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(b, eager), rep
+        _ = (b0 * 2).sum().item()   # eager work between replays
