@@ -23,6 +23,7 @@ _CTYPES = {
     "size_t": ctypes.c_size_t,
     "unsigned char": ctypes.c_ubyte,
     "gputils_gels_plan_t": ctypes.c_void_p,
+    "gputils_graph_t": ctypes.c_void_p,
     "void": None,
 }
 

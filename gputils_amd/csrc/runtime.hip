@@ -161,4 +161,46 @@ int gputils_fill_pointer_array(void **dst, const void *base, int64_t stride_byte
     return launch_check("fill_pointers_kernel");
 }
 
+struct gputils_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int gputils_graph_capture_begin(void *stream) {
+    GPUTILS_ARG(stream != nullptr, 1, "the null stream cannot be captured");
+    GPUTILS_HIP(hipStreamBeginCapture(as_stream(stream), hipStreamCaptureModeThreadLocal));
+    return GPUTILS_OK;
+}
+
+int gputils_graph_capture_end(void *stream, gputils_graph_t *graph) {
+    GPUTILS_ARG(stream != nullptr, 1, "the null stream cannot be captured");
+    GPUTILS_ARG(graph != nullptr, 2, "graph is NULL");
+    hipGraph_t g = nullptr;
+    GPUTILS_HIP(hipStreamEndCapture(as_stream(stream), &g));
+    hipGraphExec_t e = nullptr;
+    const hipError_t rc = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    if (rc != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return hip_fail(rc, "hipGraphInstantiate");
+    }
+    *graph = new gputils_graph{g, e};
+    return GPUTILS_OK;
+}
+
+int gputils_graph_launch(gputils_graph_t graph, void *stream) {
+    GPUTILS_ARG(graph != nullptr, 1, "graph is NULL");
+    GPUTILS_HIP(hipGraphLaunch(graph->exec, as_stream(stream)));
+    return GPUTILS_OK;
+}
+
+int gputils_graph_destroy(gputils_graph_t graph) {
+    if (graph == nullptr) return GPUTILS_OK;
+    const hipError_t a = hipGraphExecDestroy(graph->exec);
+    const hipError_t b = hipGraphDestroy(graph->graph);
+    delete graph;
+    if (a != hipSuccess) return hip_fail(a, "hipGraphExecDestroy");
+    if (b != hipSuccess) return hip_fail(b, "hipGraphDestroy");
+    return GPUTILS_OK;
+}
+
 }  // extern "C"

@@ -208,6 +208,17 @@ int gputils_gels_ragged_plan_destroy(gputils_gels_plan_t plan);
 int gputils_dgels_ragged(gputils_gels_plan_t plan, const double *A, double *B, int refine_steps,
                          void *stream);
 
+/* ------------------------------------------------------------ graphs --- */
+/* Capture the work enqueued on a (non-null) stream into an executable HIP
+ * graph and replay it (no reference counterpart; used for repeated
+ * Nullspace::project calls).  Between begin and end only asynchronous work
+ * may be enqueued on that stream. */
+typedef struct gputils_graph *gputils_graph_t;
+int gputils_graph_capture_begin(void *stream);
+int gputils_graph_capture_end(void *stream, gputils_graph_t *graph);
+int gputils_graph_launch(gputils_graph_t graph, void *stream);
+int gputils_graph_destroy(gputils_graph_t graph);
+
 /* ------------------------------------------------------------ SVD / rank --- */
 /* A_i = U_i diag(S_i) VT_i for tall m x n matrices (m >= n, 1 <= n <= 64):
  * replaces the per-matrix cusolverDn?gesvd loop of Svd::factorise

@@ -14,7 +14,10 @@
  *  - CholeskyBatchFactoriser::solve accepts b with any number of columns.
  *  - Svd::rank() recomputes the rank on each call instead of adding to a
  *    counter that is never reset.
+ *  - Svd handles at most 64 columns and never modifies the factorised
+ *    matrix (cuSOLVER's gesvd overwrites it).
  *  - reshape() keeps the allocated-bytes counter exact.
+ *  - StreamGraph (new) captures repeated calls into a HIP graph.
  */
 #ifndef GPUTILS_TENSOR_HPP
 #define GPUTILS_TENSOR_HPP
@@ -653,6 +656,29 @@ void DTensor<T>::addAB(const DTensor<T> &A, const DTensor<T> &B, T alpha, T beta
                                                 beta, raw(), nRA, nRA * nCB, nMat, GPUTILS_GEMM_FP32, stream()));
 }
 
+/**
+ * Batched least squares (tensor.cuh:1340-1394): for every i, b_i[0:n] <-
+ * argmin ||A_i x - b_i|| (m >= n, one right-hand side).  A_i is overwritten
+ * with its Householder QR factors.
+ */
+template <typename T>
+void DTensor<T>::leastSquaresBatched(DTensor &B) {
+    gputils_detail::require_fp<T>();
+    const size_t batchSize = numMats(), nColsB = B.numCols();
+    if (B.numRows() != m_numRows)
+        throw std::invalid_argument("[Least squares batched] rhs rows does not equal lhs rows");
+    if (nColsB != 1) throw std::invalid_argument("[Least squares batched] rhs are not vectors");
+    if (B.numMats() != batchSize)
+        throw std::invalid_argument("[Least squares batched] rhs numMats does not equal lhs numMats");
+    if (m_numCols > m_numRows)
+        throw std::invalid_argument("[Least squares batched] supports square or tall matrices only");
+    const int64_t m = (int64_t)m_numRows, n = (int64_t)m_numCols;
+    if constexpr (std::is_same_v<T, double>)
+        gpuErrChk(gputils_dgels_batched(m, n, 1, raw(), m, m * n, B.raw(), m, m, nullptr, (int64_t)batchSize, stream()));
+    if constexpr (std::is_same_v<T, float>)
+        gpuErrChk(gputils_sgels_batched(m, n, 1, raw(), m, m * n, B.raw(), m, m, nullptr, (int64_t)batchSize, stream()));
+}
+
 template <typename T>
 DTensor<T> DTensor<T>::createRandomTensor(size_t numRows, size_t numCols, size_t numMats, T low, T hi) {
     if constexpr (std::is_floating_point<T>::value) {
@@ -852,6 +878,255 @@ public:
         if constexpr (std::is_same_v<T, float>)
             gpuErrChk(gputils_spotrs_batched(n, 1, m_matrix->raw(), n, n * n, b.raw(), n, n, m_info->raw(), 1,
                                              m_matrix->stream()));
+    }
+};
+
+/* ---------------------------------------------------------------- QR --- */
+
+/**
+ * Householder QR of one tall or square matrix (tensor.cuh:1796-1995):
+ * factorise() overwrites A with R and the reflectors (LAPACK storage),
+ * leastSquares(b) overwrites b[0:n] with argmin ||A x - b||, getQR fills
+ * the thin Q (m x n) and R (n x n).  A must outlive this object.
+ */
+TEMPLATE_WITH_TYPE_T
+class QRFactoriser : public IStatus {
+private:
+    std::unique_ptr<DTensor<T>> m_householder;  ///< tau, n values
+    DTensor<T> *m_matrix;
+
+public:
+    QRFactoriser(DTensor<T> &A) : IStatus() {
+        gputils_detail::require_fp<T>();
+        if (A.numMats() > 1) throw std::invalid_argument("[QR] 3D tensors require `leastSquaresBatched`");
+        if (A.numRows() < A.numCols()) throw std::invalid_argument("[QR] Matrix A must be tall or square");
+        m_matrix = &A;
+        m_householder = std::make_unique<DTensor<T>>(A.numCols());
+    }
+
+    void factorise() {
+        const int64_t m = (int64_t)m_matrix->numRows(), n = (int64_t)m_matrix->numCols();
+        if constexpr (std::is_same_v<T, double>)
+            gpuErrChk(gputils_dgeqrf_batched(m, n, m_matrix->raw(), m, m * n, m_householder->raw(), n, m_info->raw(), 1,
+                                             m_matrix->stream()));
+        if constexpr (std::is_same_v<T, float>)
+            gpuErrChk(gputils_sgeqrf_batched(m, n, m_matrix->raw(), m, m * n, m_householder->raw(), n, m_info->raw(), 1,
+                                             m_matrix->stream()));
+    }
+
+    /** b <- Q^T b, then b[0:n] <- R^{-1} b[0:n] (tensor.cuh:1891-1927). */
+    void leastSquares(DTensor<T> &b) {
+        const int64_t m = (int64_t)m_matrix->numRows(), n = (int64_t)m_matrix->numCols();
+        if (b.numRows() != (size_t)m) throw std::invalid_argument("[QR] b must have as many rows as A");
+        if constexpr (std::is_same_v<T, double>) {
+            gpuErrChk(gputils_dormqr_batched(1, m, 1, n, m_matrix->raw(), m, m * n, m_householder->raw(), n, b.raw(), m,
+                                             m, 1, m_matrix->stream()));
+            gpuErrChk(gputils_dtrsm_upper_batched(n, 1, m_matrix->raw(), m, m * n, b.raw(), m, m, 1, m_matrix->stream()));
+        }
+        if constexpr (std::is_same_v<T, float>) {
+            gpuErrChk(gputils_sormqr_batched(1, m, 1, n, m_matrix->raw(), m, m * n, m_householder->raw(), n, b.raw(), m,
+                                             m, 1, m_matrix->stream()));
+            gpuErrChk(gputils_strsm_upper_batched(n, 1, m_matrix->raw(), m, m * n, b.raw(), m, m, 1, m_matrix->stream()));
+        }
+    }
+
+    /** Thin Q = H_0 ... H_{n-1} [I; 0] and R = triu(A[0:n, :]) (tensor.cuh:1929-1995). */
+    void getQR(DTensor<T> &Q, DTensor<T> &R) {
+        const size_t m = m_matrix->numRows(), n = m_matrix->numCols();
+        if (Q.numRows() != m || Q.numCols() != n) throw std::invalid_argument("[QR] invalid shape of Q.");
+        if (R.numRows() != n || R.numCols() != n) throw std::invalid_argument("[QR] invalid shape of R.");
+        std::vector<T> vecQ(m * n, T(0));
+        for (size_t c = 0; c < n; c++) vecQ[c + c * m] = T(1);
+        Q.upload(vecQ);
+        const int64_t mm = (int64_t)m, nn = (int64_t)n;
+        if constexpr (std::is_same_v<T, double>)
+            gpuErrChk(gputils_dormqr_batched(0, mm, nn, nn, m_matrix->raw(), mm, mm * nn, m_householder->raw(), nn, Q.raw(),
+                                             mm, mm * nn, 1, m_matrix->stream()));
+        if constexpr (std::is_same_v<T, float>)
+            gpuErrChk(gputils_sormqr_batched(0, mm, nn, nn, m_matrix->raw(), mm, mm * nn, m_householder->raw(), nn, Q.raw(),
+                                             mm, mm * nn, 1, m_matrix->stream()));
+        std::vector<T> qr(m * n);
+        m_matrix->download(qr);
+        std::vector<T> vecR(n * n, T(0));
+        for (size_t c = 0; c < n; c++)
+            for (size_t r = 0; r <= c; r++) vecR[r + c * n] = qr[r + c * m];
+        R.upload(vecR);
+    }
+};
+
+/* --------------------------------------------------------------- SVD --- */
+
+/**
+ * Batched SVD of tall matrices, A_i = U_i S_i V_i' (tensor.cuh:1500-1676).
+ * All matrices are factorised by one launch (the reference loops over
+ * them on the host).  At most 64 columns.  rank() counts sigma > epsilon
+ * afresh on every call (the reference adds to a counter that is never
+ * reset).  A is not modified by factorise(); with destroyMatrix = false a
+ * private copy is factorised, as in the reference.
+ */
+TEMPLATE_WITH_TYPE_T
+class Svd : public IStatus {
+private:
+    DTensor<T> *m_tensor = nullptr;
+    std::unique_ptr<DTensor<T>> m_copy;
+    std::shared_ptr<DTensor<T>> m_Vtr;
+    std::shared_ptr<DTensor<T>> m_S;
+    std::shared_ptr<DTensor<T>> m_U;
+    std::shared_ptr<DTensor<unsigned int>> m_rank;
+    void *m_work = nullptr;
+    size_t m_workBytes = 0;
+    bool m_computeU = false;
+
+    void checkMatrix(DTensor<T> &tensor) const {
+        if (tensor.numRows() < tensor.numCols()) throw std::invalid_argument("[svd] your matrix is fat (no offence)");
+        if (tensor.numCols() > 64) throw std::invalid_argument("[svd] at most 64 columns are supported");
+    }
+
+public:
+    Svd(DTensor<T> &mat, bool computeU = false, bool destroyMatrix = true) : IStatus(mat.numMats()) {
+        gputils_detail::require_fp<T>();
+        checkMatrix(mat);
+        if (destroyMatrix) {
+            m_tensor = &mat;
+        } else {
+            m_copy = std::make_unique<DTensor<T>>(mat);
+            m_tensor = m_copy.get();
+        }
+        m_computeU = computeU;
+        const size_t m = mat.numRows(), n = mat.numCols(), k = mat.numMats();
+        m_Vtr = std::make_shared<DTensor<T>>(n, n, k);
+        m_S = std::make_shared<DTensor<T>>(std::min(m, n), 1, k);
+        m_rank = std::make_shared<DTensor<unsigned int>>(1, 1, k, true);
+        if (computeU) m_U = std::make_shared<DTensor<T>>(m, m, k);
+        const int64_t wb = gputils_gesvd_workspace_bytes((int)m, (int)n, (int64_t)k);
+        if (wb < 0) throw std::invalid_argument("[svd] unsupported matrix size");
+        if (wb > 0) {
+            m_workBytes = (size_t)wb;
+            gpuErrChk(Session::getInstance().allocate(&m_work, m_workBytes));
+        }
+    }
+
+    ~Svd() {
+        if (m_work) {
+            gputils_free(m_work);
+            Session::getInstance().incrementAllocatedBytes(-(long long)m_workBytes);
+        }
+    }
+
+    Svd(const Svd &) = delete;
+    Svd &operator=(const Svd &) = delete;
+
+    void factorise() {
+        const int m = (int)m_tensor->numRows(), n = (int)m_tensor->numCols();
+        const int64_t k = (int64_t)m_tensor->numMats(), mn = (int64_t)m * n;
+        T *U = m_computeU ? m_U->raw() : nullptr;
+        if constexpr (std::is_same_v<T, double>)
+            gpuErrChk(gputils_dgesvd_batched(m, n, m_tensor->raw(), m, mn, m_S->raw(), n, U, m, (int64_t)m * m, m_Vtr->raw(),
+                                             n, (int64_t)n * n, m_work, m_info->raw(), nullptr, k, m_tensor->stream()));
+        if constexpr (std::is_same_v<T, float>)
+            gpuErrChk(gputils_sgesvd_batched(m, n, m_tensor->raw(), m, mn, m_S->raw(), n, U, m, (int64_t)m * m, m_Vtr->raw(),
+                                             n, (int64_t)n * n, m_work, m_info->raw(), nullptr, k, m_tensor->stream()));
+    }
+
+    DTensor<T> &singularValues() const { return *m_S; }
+
+    DTensor<T> const &rightSingularVectors() const { return *m_Vtr; }
+
+    std::optional<std::shared_ptr<DTensor<T>>> leftSingularVectors() const {
+        if (!m_computeU) return std::nullopt;
+        return m_U;
+    }
+
+    DTensor<unsigned int> const &rank(T epsilon = 1e-6) const {
+        const int n = (int)(m_S->numRows() * m_S->numCols());
+        if constexpr (std::is_same_v<T, double>)
+            gpuErrChk(gputils_dsvd_rank_batched(n, m_S->raw(), n, epsilon, m_rank->raw(), (int64_t)m_rank->numMats(),
+                                                m_tensor->stream()));
+        if constexpr (std::is_same_v<T, float>)
+            gpuErrChk(gputils_ssvd_rank_batched(n, m_S->raw(), n, epsilon, m_rank->raw(), (int64_t)m_rank->numMats(),
+                                                m_tensor->stream()));
+        return *m_rank;
+    }
+};
+
+/* --------------------------------------------------------- Nullspace --- */
+
+/**
+ * Nullspace of square or fat matrices (tensor.cuh:2008-2085): N_i spans
+ * ker A_i (zero-padded to n x n), P_i = N_i N_i'.  Built by one SVD launch
+ * of A', one rank count and one construction launch (the reference loops
+ * over the matrices on the host).
+ */
+TEMPLATE_WITH_TYPE_T
+class Nullspace {
+private:
+    std::unique_ptr<DTensor<T>> m_nullspace;
+    std::unique_ptr<DTensor<T>> m_projOp;
+
+public:
+    Nullspace(DTensor<T> &a) {
+        gputils_detail::require_fp<T>();
+        const size_t m = a.numRows(), n = a.numCols(), nMats = a.numMats();
+        if (m > n) throw std::invalid_argument("[nullspace] I was expecting a square or fat matrix");
+        m_nullspace = std::make_unique<DTensor<T>>(n, n, nMats, true);
+        m_projOp = std::make_unique<DTensor<T>>(n, n, nMats, true);
+        DTensor<T> aTranspose = a.tr();
+        Svd<T> svd(aTranspose, true);
+        svd.factorise();
+        const DTensor<unsigned int> &rank = svd.rank();
+        std::shared_ptr<DTensor<T>> U = svd.leftSingularVectors().value();
+        const int64_t nn = (int64_t)n * n;
+        if constexpr (std::is_same_v<T, double>)
+            gpuErrChk(gputils_dnullspace_build_batched((int)n, U->raw(), n, nn, rank.raw(), m_nullspace->raw(), n, nn,
+                                                       m_projOp->raw(), n, nn, (int64_t)nMats, aTranspose.stream()));
+        if constexpr (std::is_same_v<T, float>)
+            gpuErrChk(gputils_snullspace_build_batched((int)n, U->raw(), n, nn, rank.raw(), m_nullspace->raw(), n, nn,
+                                                       m_projOp->raw(), n, nn, (int64_t)nMats, aTranspose.stream()));
+        gpuErrChk(gputils_stream_synchronize(aTranspose.stream()));
+    }
+
+    DTensor<T> const &nullspace() const { return *m_nullspace; }
+
+    /** b_i <- P_i b_i in place (tensor.cuh:2081-2085). */
+    void project(DTensor<T> &b) { b.addAB(*m_projOp, b, 1, 0); }
+};
+
+/* ------------------------------------------------------------ graphs --- */
+
+/**
+ * Records the work that a callable enqueues on a Session stream into a HIP
+ * graph and replays it (no reference counterpart: used to launch repeated
+ * Nullspace::project calls without per-call launch overhead).  The
+ * callable must only enqueue work on that stream (no host synchronisation
+ * and no allocation).
+ */
+class StreamGraph {
+private:
+    gputils_graph_t m_graph = nullptr;
+    void *m_stream = nullptr;
+
+public:
+    explicit StreamGraph(size_t streamIdx = 0) : m_stream(Session::getInstance().stream(streamIdx)) {}
+    ~StreamGraph() {
+        if (m_graph) gputils_graph_destroy(m_graph);
+    }
+    StreamGraph(const StreamGraph &) = delete;
+    StreamGraph &operator=(const StreamGraph &) = delete;
+
+    template <typename F>
+    void capture(F &&f) {
+        if (m_graph) {
+            gpuErrChk(gputils_graph_destroy(m_graph));
+            m_graph = nullptr;
+        }
+        gpuErrChk(gputils_graph_capture_begin(m_stream));
+        f();
+        gpuErrChk(gputils_graph_capture_end(m_stream, &m_graph));
+    }
+
+    void launch() const {
+        if (!m_graph) throw std::logic_error("[StreamGraph] nothing captured");
+        gpuErrChk(gputils_graph_launch(m_graph, m_stream));
     }
 };
 

@@ -567,6 +567,304 @@ TEST_F(CholeskyTest, choleskyBatchNotPositiveDefinite) {  // info convention (LA
     EXPECT_EQ(2, chol.info()(0));
 }
 
+/* ================================================================ LS ====== */
+
+template <typename T>
+void tensorLeastSquares1(T epsilon) {  // X:1012-1034
+    std::vector<T> aData = {1, 2, 3, 4, 7, 8, 9, 10, 6, 8, -9, 20};
+    std::vector<T> bData = {1, 1, -1, 2, 30, -80};
+    DTensor<T> A0(aData, 2, 2, 3);
+    DTensor<T> A(A0);
+    DTensor<T> B(bData, 2, 1, 3);
+    DTensor<T> sol(B);
+    A0.leastSquaresBatched(sol);
+    DTensor<T> C(2, 1, 3);
+    C.addAB(A, sol);
+    C -= B;
+    EXPECT_LT(C.normF(), epsilon);
+}
+
+TEST_F(LeastSquaresTest, tensorLS1) {
+    tensorLeastSquares1<float>(PRECISION_LOW);
+    tensorLeastSquares1<double>(PRECISION_HIGH);
+}
+
+TEST_F(LeastSquaresTest, tensorLSPreconditions) {  // T:1344-1351
+    DTensor<double> A(3, 2, 2), fat(2, 3, 2), b(3, 1, 2), b2(3, 2, 2), bk(3, 1, 1), b4(4, 1, 2);
+    EXPECT_THROW(A.leastSquaresBatched(b4), std::invalid_argument);
+    EXPECT_THROW(A.leastSquaresBatched(b2), std::invalid_argument);
+    EXPECT_THROW(A.leastSquaresBatched(bk), std::invalid_argument);
+    DTensor<double> bf(2, 1, 2);
+    EXPECT_THROW(fat.leastSquaresBatched(bf), std::invalid_argument);
+}
+
+/* ================================================================ QR ====== */
+
+template <typename T>
+void qrFactorisationOfSize(size_t nR, size_t nC, T epsilon) {  // X:1433-1490
+    DTensor<T> temp(nR, nC);
+    DTensor<T> A = DTensor<T>::createRandomTensor(nR, nC, 1, -100, 100);
+    QRFactoriser<T> qr(temp);
+    A.deviceCopyTo(temp);
+    qr.factorise();
+    EXPECT_EQ(0, qr.info()(0));
+    DTensor<T> Q(nR, nC);
+    DTensor<T> R(nC, nC, 1, true);
+    DTensor<T> QR(nR, nC);
+    qr.getQR(Q, R);
+    EXPECT_EQ(0, qr.info()(0));
+    QR.addAB(Q, R);
+    QR -= A;
+    EXPECT_NEAR(QR.normF(), 0., epsilon);
+}
+
+TEST_F(QRTest, qrFactorisation) {
+    qrFactorisationOfSize<float>(4, 3, PRECISION_LOW);
+    qrFactorisationOfSize<double>(4, 3, PRECISION_HIGH);
+}
+
+TEST_F(QRTest, qrFactorisationTall) {
+    qrFactorisationOfSize<float>(20, 3, PRECISION_LOW);
+    qrFactorisationOfSize<double>(20, 3, PRECISION_HIGH);
+}
+
+template <typename T>
+void qrLeastSquares(T epsilon) {  // X:1496-1530
+    const size_t nR = 4, nC = 3;
+    DTensor<T> temp(nR, nC);
+    std::vector<T> vecA = {85.5638, -59.4001, -80.1992, 99.9464, 5.51393, 5.17935,
+                           6.87488, -26.7536, 36.0914, -44.3857, -32.1268, 54.8915};
+    std::vector<T> vecB = {-23.3585, -48.5744, 43.4229, -56.5081};
+    DTensor<T> A(vecA, nR, nC, 1, rowMajor);
+    DTensor<T> b(vecB, nR);
+    DTensor<T> xFull(nR);
+    DTensor<T> x(xFull, 0, 0, nC - 1);
+    DTensor<T> Ax(nR);
+    QRFactoriser<T> qr(temp);
+    A.deviceCopyTo(temp);
+    qr.factorise();
+    EXPECT_EQ(0, qr.info()(0));
+    b.deviceCopyTo(xFull);
+    qr.leastSquares(xFull);
+    EXPECT_EQ(0, qr.info()(0));
+    Ax.addAB(A, x);
+    Ax -= b;
+    EXPECT_NEAR(Ax.normF(), 80.003169364198072, epsilon);
+}
+
+TEST_F(QRTest, qrLeastSquares) {
+    qrLeastSquares<float>(PRECISION_LOW);
+    qrLeastSquares<double>(PRECISION_HIGH);
+}
+
+TEST_F(QRTest, qrPreconditions) {  // T:1811-1813, 1933-1936
+    DTensor<double> fat(2, 3), batch(3, 2, 2), A(4, 3), Qbad(4, 2), R(3, 3);
+    EXPECT_THROW(QRFactoriser<double> q(fat), std::invalid_argument);
+    EXPECT_THROW(QRFactoriser<double> q(batch), std::invalid_argument);
+    QRFactoriser<double> qr(A);
+    EXPECT_THROW(qr.getQR(Qbad, R), std::invalid_argument);
+}
+
+/* =============================================================== SVD ====== */
+
+template <typename T>
+void singularValuesComputation(T epsilon) {  // X:1058-1078
+    std::vector<T> bData = {1, 6, 6, 6, 6, 6, 6, 6, 2, 7, 7, 7, 7, 7, 7, 7, 3, 8, 8, 8, 8, 8, 8, 8};
+    DTensor<T> B(bData, 8, 3);
+    Svd<T> svd(B, true, false);
+    svd.factorise();
+    for (size_t i = 0; i < B.numMats(); i++) EXPECT_EQ(0, svd.info()(i));
+    auto S = svd.singularValues();
+    EXPECT_NEAR(32.496241123753592, S(0), epsilon);
+    EXPECT_NEAR(0.997152358903242, S(1), epsilon);
+    auto U = svd.leftSingularVectors();
+    EXPECT_TRUE(U.has_value());
+}
+
+TEST_F(SvdTest, singularValuesComputation) {
+    singularValuesComputation<float>(PRECISION_LOW);
+    singularValuesComputation<double>(PRECISION_HIGH);
+}
+
+template <typename T>
+void singularValuesMemory() {  // X:1086-1108
+    std::vector<T> bData = {1, 6, 6, 6, 6, 6, 6, 6, 2, 7, 7, 7, 7, 7, 7, 7, 3, 8, 8, 8, 8, 8, 8, 8};
+    DTensor<T> B(bData, 8, 3);
+    Svd<T> svd(B, true, false);
+    svd.factorise();
+    EXPECT_EQ(0, svd.info()(0));
+    DTensor<T> const &v1 = svd.rightSingularVectors();
+    DTensor<T> const &v2 = svd.rightSingularVectors();
+    EXPECT_EQ(&v1, &v2);
+    EXPECT_EQ(v1.raw(), v2.raw());
+    DTensor<T> const &s1 = svd.singularValues();
+    DTensor<T> const &s2 = svd.singularValues();
+    EXPECT_EQ(&s1, &s2);
+    EXPECT_EQ(s1.raw(), s2.raw());
+    auto u1 = svd.leftSingularVectors().value();
+    auto u2 = svd.leftSingularVectors().value();
+    EXPECT_EQ(u1, u2);
+    EXPECT_EQ(u1->raw(), u2->raw());
+}
+
+TEST_F(SvdTest, singularValuesMemory) {
+    singularValuesMemory<float>();
+    singularValuesMemory<double>();
+}
+
+template <typename T>
+void singularValuesMultipleMatrices(T epsilon) {  // X:1117-1160
+    std::vector<T> aData = {1, 2, 3, 4, 5, 6, 1, 1, 1, 2, 2, 2, 0, 0, 0, 0, 0, 1};
+    DTensor<T> A(aData, 3, 2, 3);
+    Svd<T> svd(A, true);
+    svd.factorise();
+    DTensor<T> const &S = svd.singularValues();
+    DTensor<T> const &V = svd.rightSingularVectors();
+    auto U = svd.leftSingularVectors().value();
+    std::vector<T> expected_v = {-0.386317703118612, -0.922365780077058, -0.922365780077058, 0.386317703118612};
+    std::vector<T> actual_v(12);
+    V.download(actual_v);
+    for (size_t i = 0; i < 4; i++) EXPECT_NEAR(expected_v[i], actual_v[i], epsilon);
+    std::vector<T> expected_s = {9.508032000695726, 0.772869635673484, 3.872983346207417, 0, 1, 0};
+    std::vector<T> actual_s(6);
+    S.download(actual_s);
+    for (size_t i = 0; i < 6; i++) EXPECT_NEAR(expected_s[i], actual_s[i], epsilon);
+    std::vector<T> expected_u = {
+        -0.428667133548626, -0.566306918848035, -0.703946704147444, 0.805963908589298, 0.112382414096594,
+        -0.581199080396110, 0.408248290463863,  -0.816496580927726, 0.408248290463863, -0.577350269189626,
+        -0.577350269189626, -0.577350269189626, 0.816496580927726,  -0.408248290463863, -0.408248290463863,
+        0.000000000000000,  -0.707106781186548, 0.707106781186547,  0, 0, -1, 1, 0, 0, 0, -1, 0};
+    std::vector<T> actual_u(27);
+    U->download(actual_u);
+    for (size_t i = 0; i < 27; i++) EXPECT_NEAR(expected_u[i], actual_u[i], epsilon);
+}
+
+TEST_F(SvdTest, singularValuesMultipleMatrices) {
+    singularValuesMultipleMatrices<float>(10 * PRECISION_LOW);
+    singularValuesMultipleMatrices<double>(PRECISION_HIGH);
+}
+
+template <typename T>
+void singularValuesRankMultipleMatrices(T epsilon) {  // X:1175-1188
+    std::vector<T> aData = {1, 4, 7, 10, 2, 5, 8, 11, 3, 6, 9, 0, 1, 4, 7, 10, 2, 5,
+                            8, 11, 3, 6, 9, 12, 1, 2, 3, 4, 2, 4, 6, 8, 3, 6, 9, 12};
+    DTensor<T> A(aData, 4, 3, 3);
+    Svd<T> svd(A);
+    svd.factorise();
+    auto rank = svd.rank(epsilon);
+    EXPECT_EQ(3u, rank(0, 0, 0));
+    EXPECT_EQ(2u, rank(0, 0, 1));
+    EXPECT_EQ(1u, rank(0, 0, 2));
+    // the count is recomputed, not accumulated (T:1602-1607 adds to it)
+    auto again = svd.rank(epsilon);
+    EXPECT_EQ(3u, again(0, 0, 0));
+}
+
+TEST_F(SvdTest, singularValuesRankMultipleMatrices) {
+    singularValuesRankMultipleMatrices<float>(PRECISION_LOW);
+    singularValuesRankMultipleMatrices<double>(PRECISION_HIGH);
+}
+
+TEST_F(SvdTest, svdPreconditions) {  // T:1518-1522
+    DTensor<double> fat(2, 3);
+    EXPECT_THROW(Svd<double> s(fat), std::invalid_argument);
+}
+
+/* ========================================================= Nullspace ====== */
+
+template <typename T>
+void computeNullspaceTensor(T epsilon) {  // X:1556-1585
+    std::vector<T> aData = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 0, 1, 2, 3, 4, 5, 6, 7, 8,
+                            9, 7, 8, 9, 1, 2, 3, 4, 2, 4, 6, 8, 3, 6, 9, 12, 1, 1, 1, 1,
+                            1, 1, 1, 1, 1, 1, 1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    DTensor<T> A(aData, 3, 4, 5);
+    Nullspace<T> ns(A);
+    DTensor<T> nA = ns.nullspace();
+    const size_t nMats = nA.numMats();
+    EXPECT_EQ(nMats, 5u);
+    for (size_t i = 0; i < nMats; i++) {
+        DTensor<T> nAi(nA, 2, i, i);
+        DTensor<T> Ai(A, 2, i, i);
+        DTensor<T> mustBeZero = Ai * nAi;
+        EXPECT_LT(mustBeZero.normF(), epsilon);
+        DTensor<T> nAiTr = nAi.tr();
+        DTensor<T> mustBeEye = nAiTr * nAi;
+        EXPECT_NEAR(1, mustBeEye(0, 0, 0), epsilon);
+        for (size_t ir = 0; ir < mustBeEye.numRows(); ir++)
+            for (size_t ic = 0; ic < mustBeEye.numCols(); ic++)
+                if (ir != ic) EXPECT_NEAR(0, mustBeEye(ir, ic, 0), epsilon);
+    }
+}
+
+TEST_F(NullspaceTest, computeNullspaceTensor) {
+    computeNullspaceTensor<float>(PRECISION_LOW);
+    computeNullspaceTensor<double>(PRECISION_HIGH);
+}
+
+template <typename T>
+void computeNullspaceTrivial() {  // X:1597-1613
+    std::vector<T> data{4, 5, 7, 4, 1, 8, 4, 5, 0, 1, 1, 1, 5, 6, 7, 9, 0, 3};
+    DTensor<T> A(data, 3, 3, 2, rowMajor);
+    Nullspace<T> nullA(A);
+    DTensor<T> N = nullA.nullspace();
+    EXPECT_EQ(N.normF(), 0);
+}
+
+TEST_F(NullspaceTest, computeNullspaceTrivial) {
+    computeNullspaceTrivial<float>();
+    computeNullspaceTrivial<double>();
+}
+
+template <typename T>
+void projectOnNullspaceTensor(T epsilon) {  // X:1624-1657
+    const size_t m = 3, n = 7;
+    std::vector<T> mat{1, -2, 3, 4, -1, -1, -1, 1, 2, -3, 4, -1, -1, -1, -1, 3, 5, -7, -1, -1, -1};
+    DTensor<T> A(m, n, 1);
+    A.upload(mat, rowMajor);
+    Nullspace<T> ns = Nullspace(A);
+    DTensor<T> N = ns.nullspace();
+    std::vector<T> vec{1, 2, 3, 4, 5, 6, 7};
+    DTensor<T> x(vec, n);
+    DTensor<T> proj(x);
+    ns.project(proj);
+    DTensor<T> error(m, 1, 1, true);
+    error.addAB(A, proj);
+    EXPECT_TRUE(error.normF() < epsilon);
+    std::vector<T> h_other{1, -2, 5, 4, 0, 0, 0};
+    DTensor<T> other(h_other, n);
+    DTensor<T> y = N * other;
+    DTensor<T> delta1 = y - proj;
+    DTensor<T> delta2 = proj - x;
+    EXPECT_LT(delta1.dotF(delta2), epsilon);
+}
+
+TEST_F(NullspaceTest, projectOnNullspaceTensor) {
+    projectOnNullspaceTensor<float>(PRECISION_LOW);
+    projectOnNullspaceTensor<double>(PRECISION_HIGH);
+}
+
+TEST_F(NullspaceTest, projectCapturedInGraph) {  // repeated project through StreamGraph
+    const size_t m = 3, n = 7;
+    std::vector<double> mat{1, -2, 3, 4, -1, -1, -1, 1, 2, -3, 4, -1, -1, -1, -1, 3, 5, -7, -1, -1, -1};
+    DTensor<double> A(m, n, 1);
+    A.upload(mat, rowMajor);
+    Nullspace<double> ns(A);
+    std::vector<double> vec{1, 2, 3, 4, 5, 6, 7};
+    DTensor<double> eager(vec, n), b(vec, n);
+    ns.project(eager);
+    StreamGraph g(b.streamIdx());
+    g.capture([&] { ns.project(b); });
+    for (int rep = 0; rep < 3; ++rep) {
+        b.upload(vec);
+        g.launch();
+        Session::getInstance().synchronizeAllStreams();
+        std::vector<double> got(n), want(n);
+        b.download(got);
+        eager.download(want);
+        for (size_t i = 0; i < n; i++) EXPECT_EQ(got[i], want[i]);
+    }
+}
+
 /* ============================================================ main ====== */
 int main(int argc, char **argv) {
     const char *filter = argc > 1 ? argv[1] : nullptr;
