@@ -15,8 +15,12 @@ launch stream; `cpu_baseline` times the FP64 oracle (oracle/oracle.c, an
 unblocked LAPACK restatement, OpenMP over matrices) on the host cores.
 
     python bench.py [--steps K] [--warmup W] [--workload NAME]
-Other workloads (gemm, cholesky_small) are for profiling and print their own
-line; they are not the headline.
+Other workloads print a line of the same shape for the other
+configurations (gemm: 16384 64x64 FP32 products; cholesky_small: FP64
+32x32 factor + solve; ls_ragged: 4096 uneven least-squares systems;
+svd_nullspace: 2048 48x96 nullspace builds and graph-replayed projections;
+copy_calibration: a 1 GiB copy for scaling rocprof byte counters).  They
+are not the headline.
 """
 from __future__ import annotations
 
@@ -44,7 +48,8 @@ def _args(argv=None):
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="cholesky_mixed",
-                    choices=["cholesky_mixed", "gemm", "cholesky_small"])
+                    choices=["cholesky_mixed", "gemm", "cholesky_small", "ls_ragged", "svd_nullspace",
+                             "copy_calibration"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison runs")
     return ap.parse_args(argv)
@@ -178,7 +183,7 @@ def bench_cholesky_mixed(a):
                                "FP32 factor + FP64 refinement to backward error 1e-12",
                    "k": k, "n": n, "nrhs": nrhs, "tol": tol},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": _traffic("posv_mixed_kernel<8>"),
                      "kernel": "posv_mixed_kernel<8> (+ FP64 fallback scan launch)",
                      "algorithmic_bytes": alg_bytes, "event_ms": ev_ms,
                      "fp_rate_tflops": alg_flops / (ev_ms * 1e-3) / 1e12},
@@ -216,38 +221,114 @@ def cpu_baseline_cholesky(A, B, min_seconds=2.0):
                       f"{B.shape[1]}-RHS solve), {dt:.2f}s of timed wall clock, {bad} failures"}
 
 
+def _line(metric, value, unit, a, ms_per_step, dtype, config, roofline, cpu, **extra):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": dtype, "data": "synthetic (seeded SplitMix64)", "config": config,
+           "roofline": roofline, "cpu_baseline": cpu}
+    out.update(extra)
+    return out
+
+
+def _roofline(bound, alg, ev_ms, kernel, unit_bytes_note):
+    if bound == "hbm":
+        achieved = alg / (ev_ms * 1e-3) / 1e9
+        peak, unit = HBM_PEAK_GBS, "GB/s"
+    else:
+        achieved = alg / (ev_ms * 1e-3) / 1e12
+        peak, unit = bound[1], "TFLOP/s"
+        bound = bound[0]
+    return {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit, "frac": achieved / peak,
+            "traffic": _traffic(kernel), "kernel": kernel, "algorithmic": alg, "algorithmic_note": unit_bytes_note,
+            "event_ms": ev_ms}
+
+
+def _traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
+    or None when no summary exists for it."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            tab = json.load(f)
+    except (OSError, ValueError):
+        return None
+    rec = tab.get("kernels", {}).get(kernel)
+    return None if rec is None else rec.get("hbm_bytes_per_launch")
+
+
+def _timed_steps(a, step):
+    import torch
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
 def bench_gemm(a):
+    """Batched C = A B, 16384 64x64 FP32 matrices: FP32 MFMA kernel (the
+    step) and the split-BF16 3-term mode, errors vs FP64 host GEMM."""
     import torch
     from gputils_amd import api, workloads
     A, B = workloads.gemm()
     k, n = A.shape[0], A.shape[1]
     dA, dB = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
     dC = torch.empty_like(dA)
-    res = {}
+    Ad = np.swapaxes(A[:256], 1, 2).astype(np.float64)
+    Bd = np.swapaxes(B[:256], 1, 2).astype(np.float64)
+    ref = Ad @ Bd
+    absAB = np.abs(Ad) @ np.abs(Bd)
+    alg = 3 * k * n * n * 4
+    modes = {}
     for mode in ("fp32", "bf16x3"):
         fn = lambda: api.gemm_batched(dA, dB, dC, mode=mode)
-        for _ in range(a.warmup):
-            fn()
-        t = statistics.median(_events_time_ms(fn, a.steps))
-        Cg = dC.cpu().numpy().astype(np.float64)
-        ref = np.matmul(np.swapaxes(B[:256].astype(np.float64), 1, 2), np.swapaxes(A[:256].astype(np.float64), 1, 2))
-        ref = np.swapaxes(ref, 1, 2)  # C^T = B^T A^T in tensor layout
-        err = np.abs(Cg[:256] - np.swapaxes(np.swapaxes(A[:256], 1, 2).astype(np.float64) @
-                                            np.swapaxes(B[:256], 1, 2).astype(np.float64), 1, 2))
-        absAB = np.swapaxes(np.abs(np.swapaxes(A[:256], 1, 2)).astype(np.float64) @
-                            np.abs(np.swapaxes(B[:256], 1, 2)).astype(np.float64), 1, 2)
-        bytes_ = 3 * k * n * n * 4
-        res[mode] = {"ms": t, "GB/s": bytes_ / (t * 1e-3) / 1e9, "hbm_frac": bytes_ / (t * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "TFLOP/s": 2 * k * n ** 3 / (t * 1e-3) / 1e12,
-                     "max_err_rel_AB": float((err / absAB).max())}
-        del ref
-    return {"metric": "batched GEMM 64x64x64 x16384 (fp32 / bf16x3)", "workload": "gemm", "results": res}
+        wall = _timed_steps(a, fn)
+        ev = statistics.median(_events_time_ms(fn, max(5, a.steps)))
+        Cg = np.swapaxes(dC[:256].cpu().numpy().astype(np.float64), 1, 2)
+        err = np.abs(Cg - ref)
+        modes[mode] = {"ms_per_step": 1e3 * wall / a.steps, "event_ms": ev,
+                       "GB/s": alg / (ev * 1e-3) / 1e9, "TFLOP/s": 2 * k * n ** 3 / (ev * 1e-3) / 1e12,
+                       "max_err_rel_absAB": float((err / absAB).max()),
+                       "max_err_rel_C": float(err.max() / np.abs(ref).max())}
+        print(f"[bench] gemm {mode}: {ev:.4f} ms {modes[mode]['GB/s']:.0f} GB/s", file=sys.stderr, flush=True)
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = _cpu_gemm(A, B)
+    m = modes["fp32"]
+    kern = "gemm_tile_kernel<float,0>"
+    return _line("batched GEMMs/s (C = A B, 64x64x64, FP32)", k / (m["ms_per_step"] * 1e-3), "GEMMs/s", a,
+                 m["ms_per_step"], "f32",
+                 {"workload": "gemm: k=16384 64x64 by 64x64 FP32, U[-1,1]", "k": k, "m": n, "n": n, "kk": n},
+                 _roofline("hbm", alg, m["event_ms"], kern, "A+B read, C written: 3*64*64*4 B per matrix"),
+                 cpu, modes=modes)
+
+
+def _cpu_gemm(A, B, min_seconds=2.0):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    pyoracle.build()
+    threads = _cpu_threads()
+    sample = 2048
+    done, dt = 0, 0.0
+    while dt < min_seconds:
+        t1 = time.perf_counter()
+        pyoracle.gemm_batched(A[:sample], B[:sample], threads)
+        dt += time.perf_counter() - t1
+        done += sample
+    return {"value": done / dt, "unit": "GEMMs/s", "cores": threads, "kind": "port",
+            "sample": f"{done} FP32 64x64x64 products (first {sample} matrices, repeated), {dt:.2f}s timed"}
 
 
 def bench_cholesky_small(a):
+    """FP64 factor + solve of 32x32 SPD systems: k = 1024 (the parity case)
+    and k = 131072 (fills the GPU; used for the roofline)."""
     import torch
     from gputils_amd import api, workloads
-    out = {}
+    res = {}
     for k in (1024, 131072):
         A, b = workloads.cholesky_small(k=k)
         dA0, db0 = torch.from_numpy(A).cuda(), torch.from_numpy(b).cuda()
@@ -260,10 +341,173 @@ def bench_cholesky_small(a):
 
         fn = lambda: api.posv_batched(dA, db, info)
         _events_time_ms(fn, a.warmup, refresh)
-        t = statistics.median(_events_time_ms(fn, a.steps, refresh))
-        bytes_ = k * (32 * 33 // 2 * 8 * 2 + 2 * 32 * 8)
-        out[str(k)] = {"ms": t, "solves_per_s": k / (t * 1e-3), "GB/s": bytes_ / (t * 1e-3) / 1e9}
-    return {"metric": "FP64 batched Cholesky factor+solve 32x32", "workload": "cholesky_small", "results": out}
+        t = statistics.median(_events_time_ms(fn, max(5, a.steps), refresh))
+        refresh()
+        fn()
+        x = db.cpu().numpy()
+        resid = float(np.max(np.linalg.norm(b[:, 0, :] - np.einsum("kij,kj->ki", A, x[:, 0, :]), axis=1) /
+                             np.linalg.norm(b[:, 0, :], axis=1)))
+        alg = k * (2 * (32 * 33 // 2) * 8 + 2 * 32 * 8)
+        res[str(k)] = {"event_ms": t, "solves_per_s": k / (t * 1e-3), "GB/s": alg / (t * 1e-3) / 1e9,
+                       "max_rel_residual": resid}
+        print(f"[bench] cholesky_small k={k}: {t:.4f} ms", file=sys.stderr, flush=True)
+    big = res["131072"]
+    cpu = None
+    if not a.no_cpu_baseline:
+        A, b = workloads.cholesky_small(k=1024)
+        cpu = cpu_baseline_cholesky(A, b)
+    return _line("batched FP64 solves/s (32x32 factor + solve)", big["solves_per_s"], "solves/s", a,
+                 big["event_ms"], "f64",
+                 {"workload": "cholesky_small: k=131072 (and 1024) SPD 32x32, A = M M^T + 32 I, 1 RHS",
+                  "k": 131072, "n": 32},
+                 _roofline("hbm", 131072 * (2 * (32 * 33 // 2) * 8 + 2 * 32 * 8), big["event_ms"],
+                           "posv_kernel<double,2,true>",
+                           "lower triangle of A read and of L written, b read and x written (8 B each)"),
+                 cpu, results=res)
+
+
+def bench_ls_ragged(a):
+    """4096 uneven least-squares systems in one plan execution."""
+    import torch
+    from gputils_amd import api, workloads
+    ms, ns, offA, offB, Af, Bf = workloads.ls_ragged()
+    plan = api.RaggedLeastSquares(ms, ns, offA, offB)
+    dA = torch.from_numpy(Af).cuda()
+    dB0 = torch.from_numpy(Bf).cuda()
+    dB = dB0.clone()
+
+    def refresh():
+        dB.copy_(dB0)
+
+    fn = lambda: plan(dA, dB)
+    _events_time_ms(fn, a.warmup, refresh)
+    evs = _events_time_ms(fn, max(5, a.steps), refresh)
+    ev = statistics.median(evs)
+    count = len(ms)
+    alg = int(8 * (np.sum(ms.astype(np.int64) * ns) + np.sum(ms) + np.sum(ns)))
+    flops = float(np.sum(2.0 * ms * ns.astype(np.float64) ** 2 - 2.0 * ns.astype(np.float64) ** 3 / 3 +
+                         4.0 * ms * ns - ns.astype(np.float64) ** 2))
+    refresh()
+    fn()
+    out = dB.cpu().numpy()
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    pyoracle.build()
+    rng = np.random.default_rng(0)
+    errs = []
+    for s in rng.choice(count, 64, replace=False):
+        m, n = int(ms[s]), int(ns[s])
+        Am = Af[offA[s]:offA[s] + m * n].reshape(n, m).T
+        ref = pyoracle.gels(Am, Bf[offB[s]:offB[s] + m]).ravel()
+        errs.append(np.linalg.norm(out[offB[s]:offB[s] + n] - ref) / np.linalg.norm(ref))
+    print(f"[bench] ls_ragged: {ev:.4f} ms, max rel err {max(errs):.2e}", file=sys.stderr, flush=True)
+    cpu = None
+    if not a.no_cpu_baseline:
+        threads = _cpu_threads()
+        sample = 1024
+        Bs = Bf.copy()
+        t1 = time.perf_counter()
+        pyoracle.gels_ragged(ms[:sample], ns[:sample], offA[:sample], offB[:sample], Af.copy(), Bs, threads)
+        dt = time.perf_counter() - t1
+        cpu = {"value": sample / dt, "unit": "systems/s", "cores": threads, "kind": "port",
+               "sample": f"first {sample} of the 4096 systems (FP64 Householder dgels restated), {dt:.2f}s"}
+    return _line("ragged least-squares systems/s (4096 systems, m in [16,512], n in [8,64])",
+                 count / (ev * 1e-3), "systems/s", a, ev, "f32+f64",
+                 {"workload": "ls_ragged: 4096 systems, n~U{8..64}, m~U{max(16,n)..512}, N(0,1), FP32 QR + "
+                              "2 FP64 augmented-system refinements", "count": count},
+                 _roofline("hbm", alg, ev, "gels_ragged_kernel (3 buckets)",
+                           "A read once, b read, x written (8 B each)"),
+                 cpu, accuracy={"max_rel_err_vs_dgels_sample64": float(max(errs))},
+                 fp_rate_tflops=flops / (ev * 1e-3) / 1e12)
+
+
+def bench_svd_nullspace(a):
+    """2048 rank-40 48x96 FP32 matrices: (i) SVD + nullspace build, (ii)
+    1000 project calls replayed from one HIP graph (the step)."""
+    import torch
+    from gputils_amd import api, workloads
+    A_t, x = workloads.svd_nullspace()
+    k, n, m = A_t.shape
+    dA = torch.from_numpy(A_t).cuda()
+    build = lambda: api.nullspace_batched(dA)
+    for _ in range(2):
+        build()
+    torch.cuda.synchronize()
+    tb = statistics.median(_events_time_ms(build, 5))
+    N, P, rank = build()
+    At = api.transpose_batched(dA)
+    S, U, VT, info, sw = api.gesvd_batched(At, compute_u=True, sweeps=True)
+    tsvd = statistics.median(_events_time_ms(lambda: api.gesvd_batched(At, compute_u=True), 5))
+    torch.cuda.synchronize()
+    b = torch.from_numpy(x).cuda()
+    reps = 1000
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        api.project(P, b)
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g, stream=s):
+            for _ in range(reps):
+                api.project(P, b)
+    torch.cuda.synchronize()
+
+    def step():
+        g.replay()
+
+    b.copy_(torch.from_numpy(x))
+    wall = _timed_steps(a, step)
+    per_call_ms = 1e3 * wall / (a.steps * reps)
+    ev = statistics.median(_events_time_ms(lambda: api.project(P, b), 20))
+    # accuracy on the projected vectors of a fresh call
+    b.copy_(torch.from_numpy(x))
+    api.project(P, b)
+    z = b.cpu().double().numpy()[:, 0, :]
+    Am = np.swapaxes(A_t.astype(np.float64), 1, 2)
+    res = np.linalg.norm(np.einsum("kij,kj->ki", Am, z), axis=1)
+    rel = float(np.max(res / np.linalg.norm(x[:, 0, :].astype(np.float64), axis=1)))
+    alg = k * (n * n * 4 + 2 * n * 4)
+    print(f"[bench] svd+nullspace build {tb:.3f} ms (svd {tsvd:.3f} ms, sweeps {int(sw.max())}); project "
+          f"{per_call_ms * 1e3:.2f} us/call in graph, {ev * 1e3:.2f} us eager; max |A P x|/|x| {rel:.2e}",
+          file=sys.stderr, flush=True)
+    cpu = None
+    if not a.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        pyoracle.build()
+        threads = _cpu_threads()
+        sample = 512
+        t1 = time.perf_counter()
+        pyoracle.nullspace_batched(A_t[:sample].astype(np.float64), 1e-6, threads)
+        dt = time.perf_counter() - t1
+        cpu = {"value": sample / dt, "unit": "nullspace builds/s", "cores": threads, "kind": "port",
+               "sample": f"SVD (one-sided Jacobi) + N + P of the first {sample} matrices in FP64, {dt:.2f}s"}
+    return _line("Nullspace::project calls/s (2048 x 96x96 P, 1000 calls per HIP graph replay)",
+                 1e3 / per_call_ms, "calls/s", a, 1e3 * wall / a.steps, "f32",
+                 {"workload": "svd_nullspace: k=2048 rank-40 48x96 FP32, sigma~U[0.1,10]; nullspace build then "
+                              "1000 graph-captured project calls per step", "k": k, "m": m, "n": n,
+                  "calls_per_step": reps},
+                 _roofline("hbm", alg, per_call_ms, "gemm_permatrix_kernel<float>",
+                           "P read (96*96*4 B) + b read and written per matrix; P (75.5 MB) fits in the 256 MB "
+                           "Infinity Cache, so graph replays can exceed the HBM rate"),
+                 cpu, build={"nullspace_build_ms": tb, "gesvd_ms": tsvd, "max_sweeps": int(sw.max()),
+                             "ranks": sorted(set(rank.cpu().tolist())), "info_nonzero": int((info != 0).sum())},
+                 accuracy={"max_rel_A_Px": rel}, project_eager_event_ms=ev)
+
+
+def bench_copy_calibration(a):
+    """Plain 1 GiB device copy (torch) for scaling the FETCH_SIZE /
+    WRITE_SIZE counters to bytes."""
+    import torch
+    nbytes = 1 << 30
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    fn = lambda: dst.copy_(src)
+    wall = _timed_steps(a, fn)
+    ev = statistics.median(_events_time_ms(fn, max(5, a.steps)))
+    return _line("device copy GB/s (counter calibration)", 2 * nbytes / (ev * 1e-3) / 1e9, "GB/s", a,
+                 1e3 * wall / a.steps, "f32", {"workload": "copy_calibration: 1 GiB read + 1 GiB written"},
+                 _roofline("hbm", 2 * nbytes, ev, "copy", "1 GiB read + 1 GiB written"), None)
 
 
 def main(argv=None):
@@ -278,8 +522,9 @@ def main(argv=None):
         return 1
     from gputils_amd import _lib
     _lib.lib()  # fails loudly when the HIP library is not built
-    fn = {"cholesky_mixed": bench_cholesky_mixed, "gemm": bench_gemm,
-          "cholesky_small": bench_cholesky_small}[a.workload]
+    fn = {"cholesky_mixed": bench_cholesky_mixed, "gemm": bench_gemm, "cholesky_small": bench_cholesky_small,
+          "ls_ragged": bench_ls_ragged, "svd_nullspace": bench_svd_nullspace,
+          "copy_calibration": bench_copy_calibration}[a.workload]
     out = fn(a)
     print(json.dumps(out), flush=True)
     return 0

@@ -355,3 +355,15 @@ void orc_dgels_ragged(int64_t count, const int32_t *m, const int32_t *n, const i
     for (int64_t s = 0; s < count; ++s)
         orc_dgels(m[s], n[s], 1, A + offA[s], m[s], B + offB[s], m[s]);
 }
+
+/* Nullspace of every m x n matrix of a batch (OpenMP over matrices). */
+int orc_nullspace_batched(int64_t m, int64_t n, const double *A, double eps, double *N, double *P,
+                          int32_t *nullity, int64_t batch, int threads) {
+    int bad = 0;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1) reduction(+ : bad)
+    for (int64_t b = 0; b < batch; ++b) {
+        nullity[b] = orc_nullspace(m, n, A + b * m * n, m, eps, N + b * n * n, P + b * n * n);
+        bad += (nullity[b] < 0) ? 1 : 0;
+    }
+    return bad;
+}

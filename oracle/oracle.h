@@ -84,6 +84,8 @@ void orc_sgemm_batched(int64_t m, int64_t n, int64_t k, const float *A,
 void orc_dgels_ragged(int64_t count, const int32_t *m, const int32_t *n,
                       const int64_t *offA, const int64_t *offB, double *A,
                       double *B, int threads);
+int orc_nullspace_batched(int64_t m, int64_t n, const double *A, double eps, double *N, double *P,
+                          int32_t *nullity, int64_t batch, int threads);
 
 #ifdef __cplusplus
 }

@@ -62,6 +62,9 @@ def lib():
         L.orc_dgemm_batched.argtypes = [_i64, _i64, _i64, _dp, _dp, _dp, _i64, ctypes.c_int]
         L.orc_sgemm_batched.argtypes = [_i64, _i64, _i64, _fp, _fp, _fp, _i64, ctypes.c_int]
         L.orc_dgels_ragged.argtypes = [_i64, _i32p, _i32p, _i64p, _i64p, _dp, _dp, ctypes.c_int]
+        L.orc_nullspace_batched.argtypes = [_i64, _i64, _dp, ctypes.c_double, _dp, _dp, _i32p, _i64,
+                                            ctypes.c_int]
+        L.orc_nullspace_batched.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -198,3 +201,16 @@ def gels_ragged(ms, ns, offA, offB, A_flat, B_flat, threads=None):
     threads = threads or os.cpu_count() or 1
     lib().orc_dgels_ragged(len(ms), _p(ms, _i32p), _p(ns, _i32p), _p(offA, _i64p),
                            _p(offB, _i64p), _p(A_flat), _p(B_flat), threads)
+
+
+def nullspace_batched(A_t, eps=1e-6, threads=None):
+    """A_t: (k, n, m) float64 tensor layout of k m x n matrices (m <= n).
+    Returns (N_t, P_t, nullity) in tensor layout (k, n, n)."""
+    A_t = np.ascontiguousarray(A_t, dtype=np.float64)
+    k, n, m = A_t.shape
+    N = np.zeros((k, n, n))
+    P = np.zeros((k, n, n))
+    nul = np.zeros(k, dtype=np.int32)
+    threads = threads or os.cpu_count() or 1
+    lib().orc_nullspace_batched(m, n, _p(A_t), eps, _p(N), _p(P), _p(nul, _i32p), k, threads)
+    return N, P, nul
