@@ -19,6 +19,7 @@
 //    Both MFMA operands of a 16x16x4 step read LDS at index lane + 64*kk.
 //  * Triangular solves use the same block inverses: per block row a 16x16
 //    matrix-vector product replaces the serial substitution.
+#include "chol_mixed.hpp"
 #include "common.hpp"
 
 namespace gputils {
@@ -820,16 +821,8 @@ int gputils_dposv_mixed_batched(int64_t n, int64_t nrhs, const double *A, int64_
     GPUTILS_ARG(iters != nullptr, 14, "iters is NULL");
     GPUTILS_ARG(info != nullptr, 15, "info is NULL");
     hipStream_t st = as_stream(stream);
-    const dim3 grid((unsigned)batch), block(chol::kThreads);
-#define GPUTILS_MIXED(NBV)                                                                      \
-    hipLaunchKernelGGL((chol::posv_mixed_kernel<NBV>), grid, block, 0, st, (int)n, (int)nrhs, A, \
-                       lda, strideA, B, ldb, strideB, X, ldx, strideX, tol, max_iter, iters, info, batch)
-    if (n <= 16) GPUTILS_MIXED(1);
-    else if (n <= 32) GPUTILS_MIXED(2);
-    else if (n <= 64) GPUTILS_MIXED(4);
-    else GPUTILS_MIXED(8);
-#undef GPUTILS_MIXED
-    int rc = launch_check("posv_mixed_kernel");
+    int rc = cholw::launch_posv_mixed((int)n, (int)nrhs, A, lda, strideA, B, ldb, strideB, X, ldx, strideX, tol,
+                                      max_iter, iters, info, batch, st);
     if (rc != GPUTILS_OK) return rc;
     // FP64 fallback for matrices flagged with iters < 0 (A, B read only)
     return chol::posv_dispatch<double, false>(n, nrhs, const_cast<double *>(A), lda, strideA, B, ldb,

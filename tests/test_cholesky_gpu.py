@@ -149,7 +149,8 @@ def test_fused_posv_matches_oracle(oracle, n, nrhs):
 
 
 # ---------------------------------------------------- mixed precision --
-@pytest.mark.parametrize("n,nrhs", [(1, 1), (8, 1), (16, 2), (33, 4), (100, 3), (128, 4), (128, 7)])
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (8, 1), (16, 2), (17, 4), (33, 4), (48, 5), (65, 1), (80, 4), (100, 3),
+                                    (112, 2), (127, 4), (128, 4), (128, 7)])
 def test_mixed_reaches_fp64_accuracy(oracle, n, nrhs):
     k = 16
     A = spd_batch(k, n, seed=300 + n, shift=0.5)
