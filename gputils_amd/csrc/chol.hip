@@ -19,7 +19,10 @@
 //    Both MFMA operands of a 16x16x4 step read LDS at index lane + 64*kk.
 //  * Triangular solves use the same block inverses: per block row a 16x16
 //    matrix-vector product replaces the serial substitution.
+#include <type_traits>
+
 #include "chol_mixed.hpp"
+#include "chol_small.hpp"
 #include "common.hpp"
 
 namespace gputils {
@@ -717,6 +720,11 @@ template <typename T, bool WRITE_L>
 int posv_dispatch(int64_t n, int64_t nrhs, T *A, int64_t lda, int64_t strideA, const T *B,
                   int64_t ldb, int64_t strideB, T *X, int64_t ldx, int64_t strideX, int *info,
                   int64_t batch, const int *mask, hipStream_t st) {
+    if constexpr (std::is_same_v<T, double>) {
+        if (n <= 32)
+            return chols::launch_posv_small((int)n, (int)nrhs, A, lda, strideA, B, ldb, strideB, X, ldx, strideX, info,
+                                            batch, mask, WRITE_L, st);
+    }
     const int64_t g = (mask != nullptr) ? (batch < 2048 ? batch : 2048) : batch;
     const dim3 grid((unsigned)g), block(kThreads);
 #define GPUTILS_POSV(NBV)                                                                     \

@@ -131,9 +131,10 @@ def test_potrs_matches_oracle(oracle, n, nrhs, dtype):
         assert np.max(np.abs(got[b].T - ref)) <= tol * max(1.0, np.abs(ref).max()), (n, nrhs, b)
 
 
-@pytest.mark.parametrize("n,nrhs", [(3, 1), (32, 1), (50, 4), (128, 4), (128, 6)])
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (3, 1), (16, 2), (17, 3), (31, 1), (32, 1), (32, 5), (50, 4), (128, 4),
+                                    (128, 6)])
 def test_fused_posv_matches_oracle(oracle, n, nrhs):
-    k = 6
+    k = 7   # odd: the n <= 32 kernel pairs matrices per wavefront
     A = spd_batch(k, n, seed=200 + n)
     B = np.random.default_rng(n).standard_normal((k, nrhs, n))
     tA, tB = dev(A), dev(B)
@@ -168,10 +169,11 @@ def test_mixed_reaches_fp64_accuracy(oracle, n, nrhs):
         assert np.max(np.abs(X[b].T - ref)) <= 1e-12 * cond * np.abs(ref).max()
 
 
-def test_mixed_falls_back_to_fp64(oracle):
+@pytest.mark.parametrize("n", [24, 48])
+def test_mixed_falls_back_to_fp64(oracle, n):
     """cond ~ 1e12 (FP32 factor unusable) and a non-SPD matrix: the FP64
     launch takes over; iters = -1 marks them."""
-    n, k = 48, 4
+    k = 4
     r = np.random.default_rng(5)
     Q, _ = np.linalg.qr(r.standard_normal((n, n)))
     A = spd_batch(k, n, seed=9)
@@ -240,3 +242,27 @@ def test_config_cholesky_mixed_full_size():
     Xp = tB2.cpu().numpy()
     assert backward_error(A, Xp, B) <= 1e-12
     assert np.max(np.abs(Xm - Xp)) <= 1e4 * 1e-12 * np.abs(Xp).max()
+
+
+@pytest.mark.parametrize("n", [5, 32])
+def test_small_fused_posv_failure_and_upper_untouched(oracle, n):
+    """n <= 32 fused path: a non-SPD matrix reports the LAPACK info of the
+    first non-positive minor, its neighbour in the same wavefront is
+    unaffected, and the strictly upper triangle of A is never written."""
+    k = 4
+    A = spd_batch(k, n, seed=77)
+    A[1][n - 1, n - 1] = -1.0          # last leading minor indefinite
+    B = np.random.default_rng(3).standard_normal((k, 1, n))
+    sentinel = np.triu(np.full((n, n), 7.5), 1)
+    M = np.array([np.tril(A[b]) + sentinel for b in range(k)])
+    tA = dev(api.from_matrices(torch.from_numpy(M)).numpy())
+    tB = dev(B)
+    info = api.posv_batched(tA, tB).cpu().tolist()
+    _, oinfo = oracle.potrf(A[1])
+    assert info[0] == 0 and info[2] == 0 and info[3] == 0 and info[1] == oinfo and oinfo > 0
+    L = api.to_matrices(tA).cpu().numpy()
+    for b in range(k):
+        assert np.array_equal(np.triu(L[b], 1), sentinel)
+    X = tB.cpu().numpy()
+    for b in (0, 2, 3):
+        assert backward_error(A[b:b + 1], X[b:b + 1], B[b:b + 1]) <= 1e-14
