@@ -415,7 +415,7 @@ def bench_ls_ragged(a):
                  count / (ev * 1e-3), "systems/s", a, ev, "f32+f64",
                  {"workload": "ls_ragged: 4096 systems, n~U{8..64}, m~U{max(16,n)..512}, N(0,1), FP32 QR + "
                               "2 FP64 augmented-system refinements", "count": count},
-                 _roofline("hbm", alg, ev, "gels_ragged_kernel (3 buckets)",
+                 _roofline("hbm", alg, ev, "gels_ragged_kernel (5 LDS-size buckets)",
                            "A read once, b read, x written (8 B each)"),
                  cpu, accuracy={"max_rel_err_vs_dgels_sample64": float(max(errs))},
                  fp_rate_tflops=flops / (ev * 1e-3) / 1e12)

@@ -48,11 +48,17 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double 
     double *cb = CB[hi ? 1 : 0];
     double *lt = LT[hi ? 1 : 0];
 
+    // every lane loads a valid element (indices clamped into the matrix) so
+    // that the loads are unconditional and all in flight together
+    const int ic = (i < n) ? i : n - 1;
     double a[N];
     static_for<0, N>([&](auto J) {
+        const int jc = (J < n) ? (int)J : n - 1;
+        a[J] = a_g[ic + (int64_t)jc * lda];
+    });
+    static_for<0, N>([&](auto J) {
         const bool in = row_in && J < n && J <= i;
-        const double v = a_g[in ? i + (int64_t)J * lda : 0];
-        a[J] = in ? v : (i == J ? 1.0 : 0.0);
+        a[J] = in ? a[J] : (i == J ? 1.0 : 0.0);
     });
 
     // factorisation; a failing half continues on a unit pivot (so no NaN
@@ -64,7 +70,11 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double 
         const bool bad = !(d0 > 0.0);
         if (bad && fail == 0) fail = K + 1;
         const double d = bad ? 1.0 : d0;
-        const double s = __builtin_sqrt(d), inv = 1.0 / s;
+        // 1/sqrt(d): hardware estimate and one Newton step (error ~ ulp^2
+        // of the estimate, below FP64 rounding)
+        double inv = __builtin_amdgcn_rsq(d);
+        inv = inv * fma(-0.5 * d, inv * inv, 1.5);
+        const double s = d * inv;
         const double t = a[K] * (inv * inv);  // A[i][k] / A[k][k]
         static_for<K + 1, N>([&](auto J) { a[J] = fma(-t, cb[J], a[J]); });
         a[K] = (i > K) ? a[K] * inv : ((i == K) ? s : 0.0);
@@ -84,17 +94,20 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double 
     const double *bb = B + (active ? b : 0) * strideB;
     double *xx = X + (active ? b : 0) * strideX;
     for (int r = 0; r < nrhs; ++r) {
+        // keeps the LDS reads below inside the loop: hoisted out of it, the
+        // loop-invariant column and diagonal reads would occupy ~128 VGPRs
+        asm volatile("" ::: "memory");
         double y = row_in ? bb[i + (int64_t)r * ldb] : 0.0;
         // forward: L y = b (row layout, y_k broadcast through the column buffer)
         static_for<0, N>([&](auto K) {
-            cb[i] = y;
+            if (i < N) cb[i] = y;  // lanes past N own no row of the buffer
             const double yk = cb[K] / lt[K * kLdT + K];
             y = (i == K) ? yk : ((i > K) ? fma(-a[K], yk, y) : y);
         });
         // backward: L^T x = y; lane i reads column i of L from LDS
         static_for<0, N>([&](auto tt) {
             constexpr int K = N - 1 - tt;
-            cb[i] = y;
+            if (i < N) cb[i] = y;  // lanes past N own no row of the buffer
             const double xk = cb[K] / lt[K * kLdT + K];
             y = (i == K) ? xk : ((i < K) ? fma(-lt[i * kLdT + K], xk, y) : y);  // L[K][i]
         });

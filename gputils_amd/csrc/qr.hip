@@ -180,6 +180,37 @@ __device__ __forceinline__ void trsv_upper_trans(const T *A, int64_t lda, int n,
 // dx = R^{-1}(d[0:n] - h), dr = Q [h; d[n:m]].  This recovers FP64
 // accuracy also for inconsistent systems, where refining x alone stalls at
 // the FP32 level.  x overwrites b[0:n].
+// FP64 r_i = base_i - sum_k A[i][k] x[k] for the rows i of this thread
+// (row-major over threads, coalesced over rows); the loads of eight
+// columns are issued together so that they are in flight at once.
+__device__ __forceinline__ void rows_minus_ax(const double *__restrict__ a, int m, int n, const double *x,
+                                              const double *base0, const double *base1, double *out) {
+    for (int i = threadIdx.x; i < m; i += kThreads) {
+        double r = base0[i] - (base1 ? base1[i] : 0.0);
+        int k = 0;
+        for (; k + 8 <= n; k += 8) {
+            double av[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) av[u] = a[i + (int64_t)(k + u) * m];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) r = fma(-av[u], x[k + u], r);
+        }
+        for (; k < n; ++k) r = fma(-a[i + (int64_t)k * m], x[k], r);
+        out[i] = r;
+    }
+}
+
+// One bucket of the ragged plan: a persistent grid walks the cost-sorted
+// item list with a static stride.  Per system: FP64 A -> FP32 (LDS), FP32
+// Householder QR with FP64 norms/dots, x0 = R^{-1} Q^T b, r0 = b - A x0 in
+// FP64, then `steps` refinements of the augmented system
+// [I A; A^T 0][r; x] = [b; 0] (Bjorck 1967): f = b - r - A x, g = -A^T r
+// in FP64, and with the FP32 factors h = R^{-T} g, d = Q^T f,
+// dx = R^{-1}(d[0:n] - h), dr = Q [h; d[n:m]].  This recovers FP64
+// accuracy also for inconsistent systems, where refining x alone stalls at
+// the FP32 level.  x overwrites b[0:n].  Global loads are batched (eight
+// per thread in flight) because one wavefront per SIMD leaves little else
+// to hide their latency.
 template <int MAXE>
 __global__ __launch_bounds__(kThreads) void gels_ragged_kernel(const RaggedItem *__restrict__ items,
                                                                int nitems, const double *__restrict__ A,
@@ -191,6 +222,7 @@ __global__ __launch_bounds__(kThreads) void gels_ragged_kernel(const RaggedItem 
     __shared__ float h[64];
     __shared__ double xd[64];
     __shared__ double rd[512];
+    __shared__ double bd[512];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
         const RaggedItem w = items[it];
@@ -198,8 +230,23 @@ __global__ __launch_bounds__(kThreads) void gels_ragged_kernel(const RaggedItem 
         const double *a = A + w.offA;
         double *bb = B + w.offB;
         __syncthreads();
-        for (int e = threadIdx.x; e < m * n; e += kThreads) S[e] = (float)a[e];
-        for (int i = threadIdx.x; i < m; i += kThreads) c[i] = (float)bb[i];
+        {
+            const int mn = m * n;
+            int e0 = threadIdx.x;
+            for (; e0 + 7 * kThreads < mn; e0 += 8 * kThreads) {
+                double v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v[u] = a[e0 + u * kThreads];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) S[e0 + u * kThreads] = (float)v[u];
+            }
+            for (; e0 < mn; e0 += kThreads) S[e0] = (float)a[e0];
+        }
+        for (int i = threadIdx.x; i < m; i += kThreads) {
+            const double v = bb[i];
+            bd[i] = v;
+            c[i] = (float)v;
+        }
         __syncthreads();
         hqr_factor<float, double>(S, m, m, n, tau, s_tau);
         if (wave == 0) {
@@ -209,24 +256,38 @@ __global__ __launch_bounds__(kThreads) void gels_ragged_kernel(const RaggedItem 
         __syncthreads();
         for (int i = threadIdx.x; i < n; i += kThreads) xd[i] = (double)c[i];
         __syncthreads();
-        // r0 = b - A x0 (FP64, A from global memory; coalesced over rows)
-        for (int i = threadIdx.x; i < m; i += kThreads) {
-            double r = bb[i];
-            for (int k = 0; k < n; ++k) r -= a[i + (int64_t)k * m] * xd[k];
-            rd[i] = r;
-        }
+        // r0 = b - A x0 (FP64, A from global memory)
+        rows_minus_ax(a, m, n, xd, bd, nullptr, rd);
         for (int s = 0; s < steps; ++s) {
             __syncthreads();
-            // f = b - r - A x  -> c (FP32)
+            // f = b - r - A x  -> c (FP32), staged through h-sized chunks of rows
             for (int i = threadIdx.x; i < m; i += kThreads) {
-                double f = bb[i] - rd[i];
-                for (int k = 0; k < n; ++k) f -= a[i + (int64_t)k * m] * xd[k];
+                double f = bd[i] - rd[i];
+                int k = 0;
+                for (; k + 8 <= n; k += 8) {
+                    double av[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) av[u] = a[i + (int64_t)(k + u) * m];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) f = fma(-av[u], xd[k + u], f);
+                }
+                for (; k < n; ++k) f = fma(-a[i + (int64_t)k * m], xd[k], f);
                 c[i] = (float)f;
             }
-            // g = -A^T r -> h (FP32); one wavefront per column
+            // g = -A^T r -> h (FP32); one wavefront per column, eight rows
+            // per lane in flight
             for (int j = wave; j < n; j += kWaves) {
+                const double *col = a + (int64_t)j * m;
                 double g = 0.0;
-                for (int i = lane; i < m; i += kWave) g += a[i + (int64_t)j * m] * rd[i];
+                int i0 = lane;
+                for (; i0 + 7 * kWave < m; i0 += 8 * kWave) {
+                    double av[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) av[u] = col[i0 + u * kWave];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) g = fma(av[u], rd[i0 + u * kWave], g);
+                }
+                for (; i0 < m; i0 += kWave) g = fma(col[i0], rd[i0], g);
                 g = wave_sum(g);
                 if (lane == 0) h[j] = (float)(-g);
             }
@@ -400,7 +461,8 @@ int gputils_gels_ragged_plan_create(gputils_gels_plan_t *plan, int64_t count, co
     GPUTILS_ARG(count == 0 || (m && n && offA && offB), 3, "size/offset arrays are NULL");
     *plan = nullptr;
     // buckets by LDS need (FP32 m*n elements); items sorted by cost m*n^2, largest first
-    static const int kBucketMax[] = {2048, 8192, 32768};
+    static const int kBucketMax[] = {2048, 4096, 8192, 16384, 32768};
+    constexpr int kNb = 5;
     std::vector<qr::RaggedItem> items((size_t)count);
     std::vector<int> order((size_t)count);
     for (int64_t i = 0; i < count; ++i) {
@@ -413,7 +475,9 @@ int gputils_gels_ragged_plan_create(gputils_gels_plan_t *plan, int64_t count, co
     }
     auto bucket_of = [&](int i) {
         const int64_t e = (int64_t)m[i] * n[i];
-        return e <= kBucketMax[0] ? 0 : (e <= kBucketMax[1] ? 1 : 2);
+        int bk = 0;
+        while (bk + 1 < kNb && e > kBucketMax[bk]) ++bk;
+        return bk;
     };
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
         const int bx = bucket_of(x), by = bucket_of(y);
@@ -427,11 +491,11 @@ int gputils_gels_ragged_plan_create(gputils_gels_plan_t *plan, int64_t count, co
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
         cus = prop.multiProcessorCount;
     int start = 0;
-    for (int bk = 2; bk >= 0; --bk) {
+    for (int bk = kNb - 1; bk >= 0; --bk) {
         int cnt = 0;
         while (start + cnt < count && bucket_of(order[start + cnt]) == bk) ++cnt;
         if (cnt > 0) {
-            const int lds = kBucketMax[bk] * 4 + 3 * 1024;
+            const int lds = kBucketMax[bk] * 4 + 12 * 1024;  // S plus the per-system vectors
             const int per_cu = std::max(1, std::min(8, 160 * 1024 / lds));
             p->buckets.push_back({kBucketMax[bk], start, cnt, std::min(cnt, cus * per_cu)});
         }
@@ -477,8 +541,12 @@ int gputils_dgels_ragged(gputils_gels_plan_t plan, const double *A, double *B, i
         const qr::RaggedItem *it = plan->d_items + bk.begin;
         if (bk.maxe == 2048)
             hipLaunchKernelGGL((qr::gels_ragged_kernel<2048>), dim3(bk.grid), dim3(qr::kThreads), 0, st, it, bk.count, A, B, refine_steps);
+        else if (bk.maxe == 4096)
+            hipLaunchKernelGGL((qr::gels_ragged_kernel<4096>), dim3(bk.grid), dim3(qr::kThreads), 0, st, it, bk.count, A, B, refine_steps);
         else if (bk.maxe == 8192)
             hipLaunchKernelGGL((qr::gels_ragged_kernel<8192>), dim3(bk.grid), dim3(qr::kThreads), 0, st, it, bk.count, A, B, refine_steps);
+        else if (bk.maxe == 16384)
+            hipLaunchKernelGGL((qr::gels_ragged_kernel<16384>), dim3(bk.grid), dim3(qr::kThreads), 0, st, it, bk.count, A, B, refine_steps);
         else
             hipLaunchKernelGGL((qr::gels_ragged_kernel<32768>), dim3(bk.grid), dim3(qr::kThreads), 0, st, it, bk.count, A, B, refine_steps);
         int rc = launch_check("gels_ragged_kernel");
