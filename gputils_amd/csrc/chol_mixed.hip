@@ -14,12 +14,13 @@
 // so they run without moving data between lanes.  Off-diagonal blocks live
 // in registers (right-looking updates); the diagonal blocks live in LDS as
 // FP32, receive their updates left-looking (A_II - sum_J U_JI^T U_JI) right
-// before they are factored by rows with lane broadcasts, and are then
-// replaced by W_I.  The backward solve (products with U, not U^T) reads
-// W_I transposed from LDS and transposes the off-diagonal blocks one at a
-// time through LDS.  Refinement residuals r = b - A x are formed in FP64 from
-// the FP64 matrix (lower triangle) in HBM/L2; the iterate x stays FP64 in
-// LDS.  Stopping test per right-hand side (LAPACK DSPOSV):
+// before they are factored (four-column panels with rank-4 MFMA updates),
+// and are then replaced by W_I.  The backward solve (products with U, not
+// U^T) reads W_I transposed from LDS and transposes the off-diagonal blocks
+// one at a time through LDS.  Refinement residuals r = b - A x are formed in
+// FP64 from the FP64 matrix in HBM, reading each 16x16 block of the lower
+// triangle once per residual; the iterate x stays FP64 in LDS.  Stopping
+// test per right-hand side (LAPACK DSPOSV):
 //   ||r||_inf <= tol (||A||_inf ||x||_inf + ||b||_inf).
 // Right-hand sides are processed four at a time (columns 0..3 of a 16x16
 // vector block; the other columns are zero).
@@ -34,6 +35,11 @@ namespace cholw {
 using f4 = f32x4;
 
 constexpr int kRhs = 4;
+// LDS tile of the residual: a 16 x 17 transpose tile followed by 16 sink
+// words that absorb the stores of lanes with nothing to add
+constexpr int kSinkOff = 16 * 17, kTileWords = 16 * 17 + 16;
+// largest leading dimension for which in-block byte offsets fit 32 bits
+constexpr int64_t kSymMaxLda = (int64_t)1 << 24;
 
 __host__ __device__ constexpr int uo(int P, int Q) { return (Q * (Q - 1)) / 2 + P; }  // block (P < Q)
 __host__ __device__ constexpr int noff(int NB) { return NB > 1 ? NB * (NB - 1) / 2 : 1; }  // array size
@@ -71,6 +77,27 @@ __device__ __forceinline__ double uniform(double v) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// Wave-wide maximum of a double by DPP: row shifts 1, 2, 4, 8 give lane 15
+// of each row the row maximum, row broadcasts 15 and 31 carry it to lane 63.
+// Lanes whose DPP source is outside the row or masked keep their own value.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_max_step(double v) {
+    const long long bits = __double_as_longlong(v);
+    const int lo = (int)(bits & 0xffffffffll), hi = (int)(bits >> 32);
+    const int olo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW_MASK, 0xF, false);
+    const int ohi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW_MASK, 0xF, false);
+    return fmax(v, __longlong_as_double(((long long)ohi << 32) | (unsigned)olo));
+}
+__device__ __forceinline__ double dpp_wave_max(double v) {
+    v = dpp_max_step<0x111, 0xF>(v);  // row_shr:1
+    v = dpp_max_step<0x112, 0xF>(v);  // row_shr:2
+    v = dpp_max_step<0x114, 0xF>(v);  // row_shr:4
+    v = dpp_max_step<0x118, 0xF>(v);  // row_shr:8
+    v = dpp_max_step<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+    v = dpp_max_step<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+    return readlane(v, 63);
+}
+
 // C + M^T N (see the file comment).
 __device__ __forceinline__ f4 mtn(const f4 &M, const f4 &N, f4 C) {
     C = __builtin_amdgcn_mfma_f32_16x16x4f32(M[0], N[0], C, 0, 0, 0);
@@ -104,48 +131,76 @@ __device__ __forceinline__ f4 transpose(const f4 &M, float *S, int lane) {
 }
 
 // Factor the symmetric block D = Ut^T Ut (Ut upper), write W = Ut^{-1}
-// column-major to `blk` and return it in the accumulator layout.  Lane c
-// (and its copies in the other lane groups) works on row c of L = Ut^T.
+// column-major to `blk` and return it in the accumulator layout.
+//
+// Blocked by four columns with rank-4 MFMA updates.  Item r of lane (g, c)
+// holds element (4g + r, c) = (c, 4g + r) of the symmetric block, so lane
+// group p holds columns 4p..4p+3 of every row c.  Panel p is factored there
+// by rows (pivot and panel rows broadcast with readlane), moved through LDS
+// into the MFMA operand layout (lane (g, c) <- L(c, 4p + g)), and one
+// 16x16x4 MFMA subtracts L_p L_p^T from the whole block (the rows already
+// factored are kept aside).  X = L^{-1} is formed the same way by block
+// rows: X_p = L_pp^{-1} E_p on lane group p, then E -= L(:, 4p..4p+3) X_p.
 // fail receives -1 or the index of the first non-positive pivot (uniform).
 __device__ __forceinline__ f4 diag_factor(const f4 &D, float *blk, int &fail) {
     const int lane = opaque_lane(), g = lane >> 4, c = lane & 15;  // lane masks stay local
-    *reinterpret_cast<f4 *>(blk + 16 * c + 4 * g) = D;
-    float row[16], invd[16];
-    const float *src = blk + c;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) row[k] = src[16 * k];
+    f4 C = D;
+    float Lop[4];  // MFMA operand of panel p: lane (g, c) holds L(c, 4p + g)
+    float inv[16];       // 1 / L(k, k), uniform
+    float lsub[4][4][4];  // L(4p + i, 4p + j), j < i, uniform
     fail = -1;
+    static_for<0, 4>([&](auto Pc) {
+        constexpr int p = Pc;
+        f4 a = C;
+        static_for<0, 4>([&](auto Kc) {
+            constexpr int k = Kc;
+            float d = readlane(a[k], 16 * p + 4 * p + k);  // pivot L(4p+k, 4p+k)^2
+            if (!(d > 0.f)) {
+                if (fail < 0) fail = 4 * p + k;
+                d = 1.f;
+            }
+            const float iv = uniform(__builtin_amdgcn_rsqf(d));
+            inv[4 * p + k] = iv;
+            const float lk = (c > 4 * p + k) ? a[k] * iv : ((c == 4 * p + k) ? d * iv : 0.f);
+            a[k] = lk;
+            // rows above 4p + j only change entries that are never read again
+            static_for<k + 1, 4>([&](auto Jc) {
+                constexpr int j = Jc;
+                a[j] = fmaf(-lk, readlane(lk, 16 * p + 4 * p + j), a[j]);
+            });
+        });
+        static_for<1, 4>([&](auto Ic) {
+            static_for<0, Ic>([&](auto Jc) {
+                constexpr int i = Ic, j = Jc;
+                lsub[p][i][j] = readlane(a[j], 16 * p + 4 * p + i);
+            });
+        });
+        // lane (g, c) <- item g of lane (p, c)
+        *reinterpret_cast<f4 *>(blk + 4 * lane) = a;
+        Lop[p] = blk[64 * p + 4 * c + g];
+        if constexpr (p < 3) C = __builtin_amdgcn_mfma_f32_16x16x4f32(-Lop[p], Lop[p], C, 0, 0, 0);
+    });
+    // E = I, then block rows of X = L^{-1}
+    f4 E, Xkeep = zero4();
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        const float d = readlane(row[k], k);
-        if (!(d > 0.f)) {
-            fail = k;
-            return D;
+    for (int r = 0; r < 4; ++r) E[r] = (4 * g + r == c) ? 1.f : 0.f;
+    static_for<0, 4>([&](auto Pc) {
+        constexpr int p = Pc;
+        f4 x;
+        x[0] = E[0] * inv[4 * p];
+        x[1] = (E[1] - lsub[p][1][0] * x[0]) * inv[4 * p + 1];
+        x[2] = (E[2] - lsub[p][2][0] * x[0] - lsub[p][2][1] * x[1]) * inv[4 * p + 2];
+        x[3] = (E[3] - lsub[p][3][0] * x[0] - lsub[p][3][1] * x[1] - lsub[p][3][2] * x[2]) * inv[4 * p + 3];
+        Xkeep = (g == p) ? x : Xkeep;
+        if constexpr (p < 3) {
+            *reinterpret_cast<f4 *>(blk + 4 * lane) = x;
+            const float bx = blk[64 * p + 4 * c + g];
+            E = __builtin_amdgcn_mfma_f32_16x16x4f32(-Lop[p], bx, E, 0, 0, 0);
         }
-        const float inv = uniform(__builtin_amdgcn_rsqf(d));
-        invd[k] = inv;
-        const float lk = (c > k) ? row[k] * inv : ((c == k) ? d * inv : 0.f);
-        row[k] = lk;
+    });
+    // W(i, j) = X(j, i) column-major: element (4g + r, c) of X at 16 (4g + r) + c
 #pragma unroll
-        for (int j = k + 1; j < 16; ++j) {
-            const float ljk = readlane(lk, j);
-            if (c >= j) row[j] -= lk * ljk;
-        }
-    }
-    // x[r] = (L^{-1})[r][c], column c of L^{-1} = row c of W
-    float x[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        float acc = (c == r) ? 1.f : 0.f;
-#pragma unroll
-        for (int p = 0; p < r; ++p) acc -= readlane(row[p], r) * x[p];
-        x[r] = acc * invd[r];
-    }
-    if (lane < 16) {
-        float *dst = blk + c;
-#pragma unroll
-        for (int r = 0; r < 16; ++r) dst[16 * r] = x[r];  // W column-major
-    }
+    for (int r = 0; r < 4; ++r) blk[16 * (4 * g + r) + c] = Xkeep[r];
     return lds_block(blk, lane);
 }
 
@@ -235,78 +290,159 @@ __device__ __forceinline__ void solve(const f4 (&U)[noff(NB)], const float *DS, 
     });
 }
 
-// Residual task t (output block Out, inner block X), row-major over the
-// NB x NB block grid.
-__host__ __device__ constexpr int task_out(int t, int NB) { return t / NB; }
-__host__ __device__ constexpr int task_x(int t, int NB) { return t % NB; }
-
-// The four FP64 elements A(16 Out + c, 16 X + 4g + r), r = 0..3, of task t
-// read from the lower triangle.  Below the diagonal the lanes of a lane
-// group read consecutive rows of a column; above it each lane reads four
-// consecutive rows of its own column.  n = 16 NB (no padding).
-template <int NB, int T>
-__device__ __forceinline__ void task_load(const double *__restrict__ a, int64_t lda, int lane, double (&v)[4]) {
-    constexpr int Out = task_out(T, NB), X = task_x(T, NB);
-    const int g = lane >> 4, c = lane & 15;
-    const int o = 16 * Out + c;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int xr = 16 * X + 4 * g + r;
-        if constexpr (X < Out) v[r] = a[o + (int64_t)xr * lda];
-        else if constexpr (X > Out) v[r] = a[xr + (int64_t)o * lda];
-        else v[r] = a[(o > xr ? o : xr) + (int64_t)(o > xr ? xr : o) * lda];
-    }
+// Sum of v over the four 16-lane rows of the wavefront (every lane receives
+// the sum of its column): row pairs (0,1), (2,3) exchanged by
+// v_permlane16_swap, then row halves by v_permlane32_swap.
+__device__ __forceinline__ double row_group_sum(double v) {
+    const long long bits = __double_as_longlong(v);
+    unsigned lo = (unsigned)(bits & 0xffffffffll), hi = (unsigned)(bits >> 32);
+    auto join = [](unsigned l, unsigned h) { return __longlong_as_double(((long long)h << 32) | l); };
+    const auto l16 = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto h16 = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    const double t = join(l16[0], h16[0]) + join(l16[1], h16[1]);
+    const long long tb = __double_as_longlong(t);
+    lo = (unsigned)(tb & 0xffffffffll);
+    hi = (unsigned)(tb >> 32);
+    const auto l32 = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto h32 = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    return join(l32[0], h32[0]) + join(l32[1], h32[1]);
 }
 
-// RD <- b - A XD (FP64, all 16*NB padded rows, 4 columns, [row][4] layout;
-// b from HBM/L2, zero beyond n and nr); when want_asum, ASUM[i] receives
-// sum_j |A(i,j)|.  Lane (g, c) accumulates output row 16 Out + c over
-// inner rows 16 X + 4g + r of block X; two xor shuffles finish each output
-// block.
+// Lower-triangle block t of the column-ordered sequence (0,0), (1,0), ...,
+// (NB-1,0), (1,1), ..., (NB-1,NB-1): (row block, column block).
+__host__ __device__ constexpr int sym_col(int t, int NB, int J = 0) {
+    return t < NB - J ? J : sym_col(t - (NB - J), NB, J + 1);
+}
+__host__ __device__ constexpr int sym_row(int t, int NB, int J = 0) {
+    return t < NB - J ? J + t : sym_row(t - (NB - J), NB, J + 1);
+}
+
+// Lane (g, c) item r of lower block (I, J): element (16I + c, 16J + 4g + r),
+// read from the lower triangle (the diagonal block is symmetrised).  The
+// block base is uniform; ob / od are the lane's byte offsets inside a
+// block below / on the diagonal.  Below the diagonal each lane group reads
+// 16 consecutive rows of a column.
+template <int NB, int T>
+__device__ __forceinline__ void sym_load(const double *__restrict__ a, int64_t lda, const uint32_t (&ob)[4],
+                                         const uint32_t (&od)[4], double (&v)[4]) {
+    constexpr int I = sym_row(T, NB), J = sym_col(T, NB);
+    const char *base = reinterpret_cast<const char *>(a + (16 * I + 16 * J * lda));
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        v[r] = *reinterpret_cast<const double *>(base + ((I > J) ? ob[r] : od[r]));
+}
+
+// RD <- b - A XD for n = 16 NB, reading each block of the lower triangle
+// once (FP64, [row][4] layout, zero beyond nr right-hand sides); when
+// want_asum, ASUM[i] receives sum_j |A(i,j)|.
 //
-// n = 16 NB: the NB^2 block tasks are unrolled and the matrix loads run four
-// tasks ahead of the arithmetic.
+// Column J of blocks is swept top to bottom.  The diagonal block and, after
+// a transpose through the LDS tile TT, every block (I, J) below it add
+// A(I,J)^T x_I into the output rows of block J, accumulated per lane and
+// reduced over the lane groups once per column.  The block itself adds
+// A(I,J) x_J into the rows of block I, reduced at once and added to RD.
 template <int NB>
-__device__ __forceinline__ void residual_full(const double *__restrict__ a, int64_t lda, const double *XD,
-                                              const double *__restrict__ bb, int64_t ldb, int nr, double *RD,
-                                              double *ASUM, bool want_asum, int lane) {
-    constexpr int NT = NB * NB, D = 4;
+__device__ __forceinline__ void residual_sym(const double *__restrict__ a, int64_t lda, const double *XD,
+                                             const double *__restrict__ bb, int64_t ldb, int nr, double *RD,
+                                             double *ASUM, bool want_asum, double *TT, int lane) {
+    constexpr int NP = 16 * NB, NT = NB * (NB + 1) / 2, D = 4;
     const int g = lane >> 4, c = lane & 15;
+    uint32_t ob[4], od[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t j = 4 * g + r, l = (uint32_t)lda;
+        ob[r] = 8u * (c + j * l);
+        od[r] = 8u * ((c > (int)j) ? c + j * l : j + c * l);
+    }
     double ring[D][4];
-    static_for<0, (D < NT ? D : NT)>([&](auto T) { task_load<NB, T>(a, lda, lane, ring[T]); });
+    static_for<0, (D < NT ? D : NT)>([&](auto T) { sym_load<NB, T>(a, lda, ob, od, ring[T]); });
+    // RD <- b (rows of this lane: e = lane + 64 m, row e / 4, column e % 4)
+    {
+        double bv[NP * kRhs / 64];
+#pragma unroll
+        for (int m = 0; m < NP * kRhs / 64; ++m) {
+            const int e = lane + 64 * m, row = e >> 2, q = e & 3;
+            bv[m] = bb[row + (int64_t)(q < nr ? q : 0) * ldb];
+        }
+#pragma unroll
+        for (int m = 0; m < NP * kRhs / 64; ++m) {
+            const int e = lane + 64 * m;
+            RD[e] = ((e & 3) < nr) ? bv[m] : 0.0;
+        }
+        if (want_asum) {
+#pragma unroll
+            for (int m = 0; m < (NP + 63) / 64; ++m)
+                if (m * 64 + 64 <= NP || lane + 64 * m < NP) ASUM[lane + 64 * m] = 0.0;
+        }
+    }
     double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0, as = 0.0;
     static_for<0, NT>([&](auto T) {
-        constexpr int Out = task_out(T, NB), X = task_x(T, NB);
+        constexpr int I = sym_row(T, NB), J = sym_col(T, NB);
         double av[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) av[r] = ring[T % D][r];
-        if constexpr (T + D < NT) task_load<NB, T + D>(a, lda, lane, ring[T % D]);
+        if constexpr (T + D < NT) sym_load<NB, T + D>(a, lda, ob, od, ring[T % D]);
         __builtin_amdgcn_sched_barrier(0);
-        const double *xb = XD + 4 * (16 * X + 4 * g);
+        double at[4];
+        if constexpr (I > J) {
+            // at[r] = element (16I + 4g + r, 16J + c): the block transposed
+#pragma unroll
+            for (int r = 0; r < 4; ++r) TT[c + 17 * (4 * g + r)] = av[r];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) at[r] = TT[4 * g + r + 17 * c];
+            // A(I,J) x_J into rows 16I + c
+            const double *xj = XD + 4 * (16 * J + 4 * g);
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const double *x = xj + 4 * r;
+                s0 += av[r] * x[0];
+                s1 += av[r] * x[1];
+                s2 += av[r] * x[2];
+                s3 += av[r] * x[3];
+            }
+            s0 = row_group_sum(s0);
+            s1 = row_group_sum(s1);
+            s2 = row_group_sum(s2);
+            s3 = row_group_sum(s3);
+            const int o = 16 * I + c;
+            const double s01 = (g & 1) ? s1 : s0, s23 = (g & 1) ? s3 : s2;
+            RD[4 * o + g] -= (g & 2) ? s23 : s01;
+            // |A| row sums, branch free (a branch per block here makes the
+            // compiler keep the blocks live across it and spill): lane
+            // group 0 adds to ASUM, the other groups to a sink
+            {
+                double sa = fabs(av[0]) + fabs(av[1]) + fabs(av[2]) + fabs(av[3]);
+                sa = row_group_sum(sa);
+                double *dst = (g == 0) ? ASUM + o : TT + kSinkOff + c;
+                *dst = *dst + (want_asum ? sa : 0.0);
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) at[r] = av[r];
+        }
+        // A(I,J)^T x_I (the diagonal block itself when I = J) into rows 16J + c
+        const double *xi = XD + 4 * (16 * I + 4 * g);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const double *x = xb + 4 * r;
-            p0 += av[r] * x[0];
-            p1 += av[r] * x[1];
-            p2 += av[r] * x[2];
-            p3 += av[r] * x[3];
-            as += fabs(av[r]);
+            const double *x = xi + 4 * r;
+            p0 += at[r] * x[0];
+            p1 += at[r] * x[1];
+            p2 += at[r] * x[2];
+            p3 += at[r] * x[3];
         }
-        if constexpr (X == NB - 1) {
-#pragma unroll
-            for (int off = 16; off <= 32; off <<= 1) {
-                p0 += __shfl_xor(p0, off, kWave);
-                p1 += __shfl_xor(p1, off, kWave);
-                p2 += __shfl_xor(p2, off, kWave);
-                p3 += __shfl_xor(p3, off, kWave);
-                as += __shfl_xor(as, off, kWave);
-            }
-            const int o = 16 * Out + c;
-            const double pg = (g == 0) ? p0 : (g == 1) ? p1 : (g == 2) ? p2 : p3;
-            const bool in = g < nr;
-            const double bvv = bb[in ? o + (int64_t)g * ldb : 0];
-            RD[4 * o + g] = (in ? bvv : 0.0) - pg;
-            if (want_asum && g == 0) ASUM[o] = as;
+        as += fabs(at[0]) + fabs(at[1]) + fabs(at[2]) + fabs(at[3]);
+        if constexpr (I == NB - 1) {
+            p0 = row_group_sum(p0);
+            p1 = row_group_sum(p1);
+            p2 = row_group_sum(p2);
+            p3 = row_group_sum(p3);
+            const int o = 16 * J + c;
+            const double p01 = (g & 1) ? p1 : p0, p23 = (g & 1) ? p3 : p2;
+            RD[4 * o + g] -= (g & 2) ? p23 : p01;
+            as = row_group_sum(as);
+            double *dst = (g == 0) ? ASUM + o : TT + kSinkOff + c;
+            *dst = *dst + (want_asum ? as : 0.0);
             p0 = p1 = p2 = p3 = as = 0.0;
         }
     });
@@ -354,13 +490,35 @@ __device__ __forceinline__ void residual_padded(const double *__restrict__ a, in
     }
 }
 
+// m[q] = max over rows of |V(row, q)| for a [row][4] LDS array of NP rows
+// (uniform result).
+template <int NP>
+__device__ __forceinline__ void col_absmax(const double *V, int ln, double (&m)[kRhs]) {
+#pragma unroll
+    for (int q = 0; q < kRhs; ++q) m[q] = 0.0;
+#pragma unroll
+    for (int row0 = 0; row0 < NP; row0 += 64) {
+        const int row = row0 + ln;
+        if (row0 + 64 <= NP || row < NP) {
+            const double *v = V + 4 * row;
+#pragma unroll
+            for (int q = 0; q < kRhs; ++q) m[q] = fmax(m[q], fabs(v[q]));
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kRhs; ++q) m[q] = dpp_wave_max(m[q]);
+}
+
 template <int NB>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void posv_mixed_wave_kernel(
     int n, int nrhs, const double *__restrict__ A, int64_t lda, int64_t strideA, const double *__restrict__ B,
     int64_t ldb, int64_t strideB, double *__restrict__ X, int64_t ldx, int64_t strideX, double tol, int max_iter,
     int *iters, int *info, int64_t batch) {
     constexpr int NP = 16 * NB;
-    __shared__ __attribute__((aligned(16))) float S[256];
+    // S: transposes of the solve (256 floats); TS as doubles: the transpose
+    // tile of the residual (the two uses never overlap in time)
+    __shared__ __attribute__((aligned(16))) double TS[kTileWords];
+    float *S = reinterpret_cast<float *>(TS);
     __shared__ __attribute__((aligned(16))) float DS[256 * NB];
     __shared__ __attribute__((aligned(16))) double XD[NP * kRhs];
     __shared__ __attribute__((aligned(16))) double RD[NP * kRhs];
@@ -393,13 +551,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void po
             XD[e] = 0.0;
         }
         double bn[kRhs];
-#pragma unroll
-        for (int q = 0; q < kRhs; ++q) {
-            double m = 0.0;
-#pragma unroll 1
-            for (int row = lane; row < NP; row += 64) m = fmax(m, fabs(RD[4 * row + q]));
-            bn[q] = uniform(wave_max(m));
-        }
+        col_absmax<NP>(RD, lane, bn);
         int it = 0;
         bool converged = false;
         for (;;) {
@@ -422,28 +574,26 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void po
                     for (int r = 0; r < 4; ++r) dst[64 * Ic + 4 * r] += (double)Y[Ic][r];
                 });
             }
-            if (n == 16 * NB) residual_full<NB>(a, lda, XD, bb, ldb, nr, RD, ASUM, anorm == 0.0, ln);
+            // orders the float accesses of the solve before the double
+            // accesses of the residual to the same LDS words
+            asm volatile("" ::: "memory");
+            if (n == 16 * NB && lda <= kSymMaxLda) residual_sym<NB>(a, lda, XD, bb, ldb, nr, RD, ASUM, anorm == 0.0, TS, ln);
             else residual_padded<NB>(a, lda, n, XD, bb, ldb, nr, RD, ASUM, anorm == 0.0, ln);
             if (anorm == 0.0) {
                 double m = 0.0;
-#pragma unroll 1
-                for (int row = ln; row < NP; row += 64) m = fmax(m, ASUM[row]);
-                anorm = uniform(wave_max(m));
+#pragma unroll
+                for (int row0 = 0; row0 < NP; row0 += 64)
+                    if (row0 + 64 <= NP || row0 + ln < NP) m = fmax(m, ASUM[row0 + ln]);
+                anorm = dpp_wave_max(m);
                 if (anorm == 0.0) anorm = 1e-300;  // A = 0 is not SPD; keep the test finite
             }
+            double rm[kRhs], xm[kRhs];
+            col_absmax<NP>(RD, ln, rm);
+            col_absmax<NP>(XD, ln, xm);
             bool ok = true;
 #pragma unroll
-            for (int q = 0; q < kRhs; ++q) {
-                double rm = 0.0, xm = 0.0;
-#pragma unroll 1
-                for (int row = ln; row < NP; row += 64) {
-                    rm = fmax(rm, fabs(RD[4 * row + q]));
-                    xm = fmax(xm, fabs(XD[4 * row + q]));
-                }
-                rm = wave_max(rm);
-                xm = wave_max(xm);
-                if (q < nr) ok = ok && (rm <= tol * (anorm * xm + bn[q]));
-            }
+            for (int q = 0; q < kRhs; ++q)
+                if (q < nr) ok = ok && (rm[q] <= tol * (anorm * xm[q] + bn[q]));
             if (ok) {
                 converged = true;
                 break;
