@@ -183,8 +183,8 @@ def bench_cholesky_mixed(a):
                                "FP32 factor + FP64 refinement to backward error 1e-12",
                    "k": k, "n": n, "nrhs": nrhs, "tol": tol},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": _traffic("posv_mixed_kernel<8>"),
-                     "kernel": "posv_mixed_kernel<8> (+ FP64 fallback scan launch)",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": _traffic("posv_mixed_wave_kernel<8>"),
+                     "kernel": "posv_mixed_wave_kernel<8> (+ FP64 fallback scan launch)",
                      "algorithmic_bytes": alg_bytes, "event_ms": ev_ms,
                      "fp_rate_tflops": alg_flops / (ev_ms * 1e-3) / 1e12},
         "cpu_baseline": cpu,
@@ -230,7 +230,7 @@ def _line(metric, value, unit, a, ms_per_step, dtype, config, roofline, cpu, **e
     return out
 
 
-def _roofline(bound, alg, ev_ms, kernel, unit_bytes_note):
+def _roofline(bound, alg, ev_ms, kernel, unit_bytes_note, traffic_key=None):
     if bound == "hbm":
         achieved = alg / (ev_ms * 1e-3) / 1e9
         peak, unit = HBM_PEAK_GBS, "GB/s"
@@ -239,21 +239,26 @@ def _roofline(bound, alg, ev_ms, kernel, unit_bytes_note):
         peak, unit = bound[1], "TFLOP/s"
         bound = bound[0]
     return {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit, "frac": achieved / peak,
-            "traffic": _traffic(kernel), "kernel": kernel, "algorithmic": alg, "algorithmic_note": unit_bytes_note,
+            "traffic": _traffic(traffic_key or kernel), "kernel": kernel, "algorithmic": alg, "algorithmic_note": unit_bytes_note,
             "event_ms": ev_ms}
 
 
 def _traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
-    or None when no summary exists for it."""
+    or None when no summary exists for it.  A name ending in '*' sums every
+    kernel whose name starts with the rest (one launch of each per step)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             tab = json.load(f)
     except (OSError, ValueError):
         return None
-    rec = tab.get("kernels", {}).get(kernel)
+    kernels = tab.get("kernels", {})
+    if kernel.endswith("*"):
+        vals = [v.get("hbm_bytes_per_launch") for k, v in kernels.items() if k.startswith(kernel[:-1])]
+        return sum(vals) if vals and None not in vals else None
+    rec = kernels.get(kernel)
     return None if rec is None else rec.get("hbm_bytes_per_launch")
 
 
@@ -299,7 +304,7 @@ def bench_gemm(a):
     if not a.no_cpu_baseline:
         cpu = _cpu_gemm(A, B)
     m = modes["fp32"]
-    kern = "gemm_tile_kernel<float,0>"
+    kern = "gemm_tiled_kernel<float,0>"
     return _line("batched GEMMs/s (C = A B, 64x64x64, FP32)", k / (m["ms_per_step"] * 1e-3), "GEMMs/s", a,
                  m["ms_per_step"], "f32",
                  {"workload": "gemm: k=16384 64x64 by 64x64 FP32, U[-1,1]", "k": k, "m": n, "n": n, "kk": n},
@@ -361,7 +366,7 @@ def bench_cholesky_small(a):
                  {"workload": "cholesky_small: k=131072 (and 1024) SPD 32x32, A = M M^T + 32 I, 1 RHS",
                   "k": 131072, "n": 32},
                  _roofline("hbm", 131072 * (2 * (32 * 33 // 2) * 8 + 2 * 32 * 8), big["event_ms"],
-                           "posv_kernel<double,2,true>",
+                           "posv_small_kernel<32,true>",
                            "lower triangle of A read and of L written, b read and x written (8 B each)"),
                  cpu, results=res)
 
@@ -415,8 +420,8 @@ def bench_ls_ragged(a):
                  count / (ev * 1e-3), "systems/s", a, ev, "f32+f64",
                  {"workload": "ls_ragged: 4096 systems, n~U{8..64}, m~U{max(16,n)..512}, N(0,1), FP32 QR + "
                               "2 FP64 augmented-system refinements", "count": count},
-                 _roofline("hbm", alg, ev, "gels_ragged_kernel (5 LDS-size buckets)",
-                           "A read once, b read, x written (8 B each)"),
+                 _roofline("hbm", alg, ev, "gels_ragged_kernel<*> (one launch per LDS-size bucket)",
+                           "A read once, b read, x written (8 B each)", traffic_key="gels_ragged_kernel<*"),
                  cpu, accuracy={"max_rel_err_vs_dgels_sample64": float(max(errs))},
                  fp_rate_tflops=flops / (ev * 1e-3) / 1e12)
 
