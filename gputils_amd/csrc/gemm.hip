@@ -18,6 +18,9 @@
 // makes the in-place product of Nullspace::project (tensor.cuh:2084) safe.
 #include "common.hpp"
 
+#include <cstdint>
+#include <type_traits>
+
 namespace gputils {
 namespace gemm {
 
@@ -166,6 +169,124 @@ __global__ __launch_bounds__(kThreads) void gemm_tiled_kernel(int m, int n, int 
     }
 }
 
+// Register-direct kernel for m, n <= 64 and k in {16, 32, 48, 64} (FP32):
+// one workgroup per matrix, wave w computes the 32x32 quadrant (wm, wn) of
+// C.  The reduction index is permuted so that lane (g, c) owns
+// k = 16g + t, t = 0..15: its B elements are 16 consecutive floats of one
+// column (four 16-byte loads) and its A elements are row wm + 16x + c of 16
+// columns (loads coalesced across the 16 lanes of a group).  FP32 mode
+// issues one 16x16x4 MFMA per t (the four lane groups supply
+// k = 16g + t); BF16X3 mode one 16x16x32 step per 8 consecutive t.  No LDS
+// and no staging: every load is in flight at once, and C leaves the
+// accumulators as 16-byte stores of four consecutive rows when vec_c.
+template <int MODE>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MODE == GPUTILS_GEMM_BF16X3 ? 6 : 7))) void gemm_reg_kernel(int m, int n, int k, float alpha,
+                                                            const float *__restrict__ A, int64_t lda,
+                                                            int64_t strideA, const float *__restrict__ B,
+                                                            int64_t ldb, int64_t strideB, float beta, float *C,
+                                                            int64_t ldc, int64_t strideC, int vec_c,
+                                                            int64_t batch) {
+    const int64_t bidx = blockIdx.x + (int64_t)blockIdx.y * 65535;
+    if (bidx >= batch) return;  // uniform per workgroup
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, c = lane & 15;
+    const int wm = 32 * (wave & 1), wn = 32 * (wave >> 1);
+    const float *a = A + bidx * strideA;
+    const float *b = B + bidx * strideB;
+    float *cm = C + bidx * strideC;
+    const bool kin = 16 * g < k;  // k is a multiple of 16: a lane group is all in or all out
+    // loads: addresses clamped into the matrix, values zeroed outside it
+    float av[2][16], bv[2][16];
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+        const int row = wm + 16 * x + c;
+        const int64_t rc = row < m ? row : m - 1;
+        const int64_t k0 = kin ? 16 * g : 0;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) av[x][t] = a[rc + (k0 + t) * lda];
+        const bool in = kin && row < m;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) av[x][t] = in ? av[x][t] : 0.f;
+    }
+#pragma unroll
+    for (int y = 0; y < 2; ++y) {
+        const int col = wn + 16 * y + c;
+        const int64_t cc = col < n ? col : n - 1;
+        const float4 *p = reinterpret_cast<const float4 *>(b + cc * ldb + (kin ? 16 * g : 0));
+        const bool in = kin && col < n;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float4 v = p[q];
+            bv[y][4 * q + 0] = in ? v.x : 0.f;
+            bv[y][4 * q + 1] = in ? v.y : 0.f;
+            bv[y][4 * q + 2] = in ? v.z : 0.f;
+            bv[y][4 * q + 3] = in ? v.w : 0.f;
+        }
+    }
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] = f32x4{0.f, 0.f, 0.f, 0.f};
+    if constexpr (MODE == GPUTILS_GEMM_BF16X3) {
+#pragma unroll
+        for (int s8 = 0; s8 < 2; ++s8) {
+            bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+            for (int x = 0; x < 2; ++x) {
+                split_bf16(&av[x][8 * s8], ah[x], al[x]);
+                split_bf16(&bv[x][8 * s8], bh[x], bl[x]);
+            }
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y) {
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[x], bh[y], acc[x][y], 0, 0, 0);
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bl[y], acc[x][y], 0, 0, 0);
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bh[y], acc[x][y], 0, 0, 0);
+                }
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[x][t], bv[y][t], acc[x][y], 0, 0, 0);
+    }
+    // every wave has consumed its A and B before any C element is written
+    // (C may alias A)
+    __syncthreads();
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            const int row0 = wm + 16 * x + 4 * g, col = wn + 16 * y + c;
+            if (col >= n) continue;
+            float *dst = cm + row0 + (int64_t)col * ldc;
+            if (vec_c && row0 + 3 < m) {
+                float4 v = {alpha * acc[x][y][0], alpha * acc[x][y][1], alpha * acc[x][y][2], alpha * acc[x][y][3]};
+                if (beta != 0.f) {
+                    const float4 o = *reinterpret_cast<const float4 *>(dst);
+                    v.x += beta * o.x;
+                    v.y += beta * o.y;
+                    v.z += beta * o.z;
+                    v.w += beta * o.w;
+                }
+                *reinterpret_cast<float4 *>(dst) = v;
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    if (row0 + r >= m) continue;
+                    float v = alpha * acc[x][y][r];
+                    if (beta != 0.f) v += beta * dst[r];
+                    dst[r] = v;
+                }
+            }
+        }
+}
+
 // Per-matrix kernel: stage B_i (k x n) in LDS, then each thread group
 // computes C elements with a split-K dot product reduced through LDS.
 template <typename T>
@@ -240,6 +361,18 @@ int run(int64_t m, int64_t n, int64_t k, T alpha, const T *A, int64_t lda, int64
                            (int)n, (int)k, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc,
                            strideC, split, batch);
         return launch_check("gemm_permatrix_kernel");
+    }
+    if constexpr (std::is_same<T, float>::value) {
+        const bool b_vec = (reinterpret_cast<uintptr_t>(B) % 16 == 0) && ldb % 4 == 0 && strideB % 4 == 0;
+        if (m <= kTM && n <= kTN && k > 0 && k <= 64 && k % 16 == 0 && b_vec) {
+            const int vec_c = (reinterpret_cast<uintptr_t>(C) % 16 == 0) && ldc % 4 == 0 && strideC % 4 == 0;
+            const unsigned gx = (unsigned)(batch < 65535 ? batch : 65535);
+            const unsigned gy = (unsigned)ceil_div(batch, 65535);
+            hipLaunchKernelGGL((gemm_reg_kernel<MODE>), dim3(gx, gy), dim3(kThreads), 0, st, (int)m, (int)n,
+                               (int)k, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc, strideC, vec_c,
+                               batch);
+            return launch_check("gemm_reg_kernel");
+        }
     }
     const int tiles_m = (int)ceil_div(m, kTM), tiles_n = (int)ceil_div(n, kTN);
     const int64_t tiles = (int64_t)tiles_m * tiles_n;
