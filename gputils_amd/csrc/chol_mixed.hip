@@ -241,7 +241,12 @@ __device__ __forceinline__ void load_matrix(const double *__restrict__ a, int64_
 }
 
 // Blocked factorisation: off-diagonal blocks right-looking in registers,
-// diagonal blocks left-looking from LDS; DS[J] ends up holding W_J.
+// diagonal blocks left-looking from LDS; DS[J] ends up holding W_J.  Each
+// block product of an update is accumulated from zero and then added to
+// its target: the 16x16x4 MFMA rounds after every product, so
+// accumulating straight into the Schur complement would round 16 times
+// per block product at the complement's magnitude, which doubles the
+// factor's error and with it the refinement steps needed.
 // Returns 0 or the 1-based failing row.
 template <int NB>
 __device__ __forceinline__ int factor(f4 (&U)[noff(NB)], float *DS) {
@@ -253,7 +258,7 @@ __device__ __forceinline__ int factor(f4 (&U)[noff(NB)], float *DS) {
     for (int J = 0; J < NB; ++J) {
         f4 d = lds_block(DS + 256 * J, opaque_lane());
         static_for<1, NB>([&](auto Jc) {
-            if (J == Jc) static_for<0, Jc>([&](auto Ic) { d = mtn(-U[uo(Ic, Jc)], U[uo(Ic, Jc)], d); });
+            if (J == Jc) static_for<0, Jc>([&](auto Ic) { d += mtn(-U[uo(Ic, Jc)], U[uo(Ic, Jc)], zero4()); });
         });
         int fail;
         const f4 w = diag_factor(d, DS + 256 * J, fail);
@@ -265,7 +270,7 @@ __device__ __forceinline__ int factor(f4 (&U)[noff(NB)], float *DS) {
                 static_for<JJ + 1, NB>([&](auto Ic) {
                     constexpr int I = Ic;
                     const f4 m = -U[uo(JJ, I)];
-                    static_for<I + 1, NB>([&](auto Kc) { U[uo(I, Kc)] = mtn(m, U[uo(JJ, Kc)], U[uo(I, Kc)]); });
+                    static_for<I + 1, NB>([&](auto Kc) { U[uo(I, Kc)] += mtn(m, U[uo(JJ, Kc)], zero4()); });
                 });
             }
         });
