@@ -725,7 +725,9 @@ int posv_dispatch(int64_t n, int64_t nrhs, T *A, int64_t lda, int64_t strideA, c
             return chols::launch_posv_small((int)n, (int)nrhs, A, lda, strideA, B, ldb, strideB, X, ldx, strideX, info,
                                             batch, mask, WRITE_L, st);
     }
-    const int64_t g = (mask != nullptr) ? (batch < 2048 ? batch : 2048) : batch;
+    // masked (fallback) launches scan the mask with a grid-stride loop: two
+    // workgroups per CU fill the GPU at this kernel's LDS size
+    const int64_t g = (mask != nullptr) ? (batch < 512 ? batch : 512) : batch;
     const dim3 grid((unsigned)g), block(kThreads);
 #define GPUTILS_POSV(NBV)                                                                     \
     hipLaunchKernelGGL((posv_kernel<T, NBV, WRITE_L>), grid, block, 0, st, (int)n, (int)nrhs, A, \
