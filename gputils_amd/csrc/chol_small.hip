@@ -10,7 +10,8 @@
 // updates its row with A[i][j] -= A[i][k] A[j][k] / A[k][k].  The matrix is
 // padded with the identity to N = 16 or 32 rows so the unrolled code needs
 // no bounds checks.  The forward solve uses the row layout; the backward
-// solve (L^T x = y) reads L by columns from an LDS copy.
+// solve (L^T x = y) reads L by columns from an LDS copy.  Both multiply by
+// the reciprocal pivots kept from the factorisation instead of dividing.
 #include "chol_small.hpp"
 #include "common.hpp"
 
@@ -36,6 +37,7 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double 
                                                         const int *mask) {
     __shared__ __attribute__((aligned(16))) double LT[2][N * kLdT];
     __shared__ __attribute__((aligned(16))) double CB[2][N];  // per-half broadcast buffer
+    __shared__ double RI[2][N];                               // per-half 1 / L(k, k)
     const int lane = threadIdx.x, i = lane & 31;
     const bool hi = lane >= 32;
     const int64_t b = 2 * (int64_t)blockIdx.x + (hi ? 1 : 0);
@@ -47,6 +49,7 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double 
     double *a_g = A + (active ? b : 0) * strideA;
     double *cb = CB[hi ? 1 : 0];
     double *lt = LT[hi ? 1 : 0];
+    double *ri = RI[hi ? 1 : 0];
 
     // every lane loads a valid element (indices clamped into the matrix) so
     // that the loads are unconditional and all in flight together
@@ -66,6 +69,9 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double 
     int fail = 0;
     static_for<0, N>([&](auto K) {
         if (i < N) cb[i] = a[K];
+        // the broadcast reads below must see the other lanes' writes: lanes
+        // that wrote nothing would otherwise reuse an earlier read of cb
+        asm volatile("" ::: "memory");
         const double d0 = cb[K];
         const bool bad = !(d0 > 0.0);
         if (bad && fail == 0) fail = K + 1;
@@ -75,6 +81,7 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double 
         double inv = __builtin_amdgcn_rsq(d);
         inv = inv * fma(-0.5 * d, inv * inv, 1.5);
         const double s = d * inv;
+        ri[K] = inv;  // every lane of the half holds the same value
         const double t = a[K] * (inv * inv);  // A[i][k] / A[k][k]
         static_for<K + 1, N>([&](auto J) { a[J] = fma(-t, cb[J], a[J]); });
         a[K] = (i > K) ? a[K] * inv : ((i == K) ? s : 0.0);
@@ -101,14 +108,16 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double 
         // forward: L y = b (row layout, y_k broadcast through the column buffer)
         static_for<0, N>([&](auto K) {
             if (i < N) cb[i] = y;  // lanes past N own no row of the buffer
-            const double yk = cb[K] / lt[K * kLdT + K];
+            asm volatile("" ::: "memory");
+            const double yk = cb[K] * ri[K];
             y = (i == K) ? yk : ((i > K) ? fma(-a[K], yk, y) : y);
         });
         // backward: L^T x = y; lane i reads column i of L from LDS
         static_for<0, N>([&](auto tt) {
             constexpr int K = N - 1 - tt;
             if (i < N) cb[i] = y;  // lanes past N own no row of the buffer
-            const double xk = cb[K] / lt[K * kLdT + K];
+            asm volatile("" ::: "memory");
+            const double xk = cb[K] * ri[K];
             y = (i == K) ? xk : ((i < K) ? fma(-lt[i * kLdT + K], xk, y) : y);  // L[K][i]
         });
         if (ok && i < n) xx[i + (int64_t)r * ldx] = y;
