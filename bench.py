@@ -495,7 +495,7 @@ def bench_svd_nullspace(a):
                  {"workload": "svd_nullspace: k=2048 rank-40 48x96 FP32, sigma~U[0.1,10]; nullspace build then "
                               "1000 graph-captured project calls per step", "k": k, "m": m, "n": n,
                   "calls_per_step": reps},
-                 _roofline("hbm", alg, per_call_ms, "gemm_permatrix_kernel<float>",
+                 _roofline("hbm", alg, per_call_ms, "gemv_quad_kernel",
                            "P read (96*96*4 B) + b read and written per matrix; P (75.5 MB) fits in the 256 MB "
                            "Infinity Cache, so graph replays can exceed the HBM rate"),
                  cpu, build={"nullspace_build_ms": tb, "gesvd_ms": tsvd, "max_sweeps": int(sw.max()),

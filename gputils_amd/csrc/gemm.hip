@@ -287,6 +287,79 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MODE =
         }
 }
 
+// Batched GEMV y_i <- alpha A_i x_i + beta y_i (FP32, m % 4 == 0,
+// m / 4 <= kThreads, 16-byte aligned columns), y may alias x (the in-place
+// Nullspace::project, tensor.cuh:2084).  One workgroup per matrix: x is
+// staged in LDS first; thread t owns row quad q = t % (m/4) of column
+// group jg = t / (m/4) and reads the columns jg, jg + G, ... of its quad
+// as 16-byte loads, all in flight together (consecutive threads read
+// consecutive quads of a column, so each column is one contiguous read);
+// the G partial sums of a quad are reduced through LDS and written after
+// the last barrier.
+__global__ __launch_bounds__(kThreads) void gemv_quad_kernel(int m, int k, float alpha, const float *__restrict__ A,
+                                                             int64_t lda, int64_t strideA, const float *x,
+                                                             int64_t strideX, float beta, float *y,
+                                                             int64_t strideY, int64_t batch) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char dyn[];
+    float *xs = reinterpret_cast<float *>(dyn);                   // k
+    f32x4 *part = reinterpret_cast<f32x4 *>(xs + ((k + 3) & ~3));  // kThreads quads
+    const int64_t bidx = blockIdx.x + (int64_t)blockIdx.y * 65535;
+    if (bidx >= batch) return;  // uniform per workgroup
+    const float *a = A + bidx * strideA;
+    const float *xb = x + bidx * strideX;
+    float *yb = y + bidx * strideY;
+    const int t = threadIdx.x, rq = m >> 2, G = kThreads / rq;
+    const int q = t % rq, jg = t / rq;
+    for (int j = t; j < k; j += kThreads) xs[j] = xb[j];
+    __syncthreads();
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    if (jg < G) {
+        constexpr int U = 8;  // loads in flight per thread
+        int j = jg;
+        for (; j + (U - 1) * G < k; j += U * G) {
+            f32x4 v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                v[u] = *reinterpret_cast<const f32x4 *>(a + 4 * q + (int64_t)(j + u * G) * lda);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const float xj = xs[j + u * G];
+                acc[0] = fmaf(v[u][0], xj, acc[0]);
+                acc[1] = fmaf(v[u][1], xj, acc[1]);
+                acc[2] = fmaf(v[u][2], xj, acc[2]);
+                acc[3] = fmaf(v[u][3], xj, acc[3]);
+            }
+        }
+        for (; j < k; j += G) {
+            const f32x4 v = *reinterpret_cast<const f32x4 *>(a + 4 * q + (int64_t)j * lda);
+            const float xj = xs[j];
+            acc[0] = fmaf(v[0], xj, acc[0]);
+            acc[1] = fmaf(v[1], xj, acc[1]);
+            acc[2] = fmaf(v[2], xj, acc[2]);
+            acc[3] = fmaf(v[3], xj, acc[3]);
+        }
+    }
+    part[t] = acc;
+    __syncthreads();
+    if (t < rq) {
+        f32x4 s = part[t];
+        for (int g = 1; g < G; ++g) {
+            const f32x4 p = part[t + g * rq];
+            s[0] += p[0];
+            s[1] += p[1];
+            s[2] += p[2];
+            s[3] += p[3];
+        }
+        float *dst = yb + 4 * t;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            float v = alpha * s[r];
+            if (beta != 0.f) v += beta * dst[r];
+            dst[r] = v;
+        }
+    }
+}
+
 // Per-matrix kernel: stage B_i (k x n) in LDS, then each thread group
 // computes C elements with a split-K dot product reduced through LDS.
 template <typename T>
@@ -343,6 +416,20 @@ int run(int64_t m, int64_t n, int64_t k, T alpha, const T *A, int64_t lda, int64
     if (alias && (ldb != ldc || strideB != strideC)) {
         set_error("gemm: C may alias B only with identical ld and stride");
         return -10;
+    }
+    if constexpr (std::is_same<T, float>::value) {
+        // matrix-vector products (the in-place project among them)
+        const bool quad_ok = n == 1 && k > 0 && m % 4 == 0 && m / 4 <= kThreads && lda % 4 == 0 &&
+                             strideA % 4 == 0 && reinterpret_cast<uintptr_t>(A) % 16 == 0 &&
+                             (size_t)k * sizeof(T) + 16 * kThreads + 16 <= 64 * 1024;
+        if (quad_ok) {
+            const size_t lds = sizeof(T) * (size_t)((k + 3) & ~3) + 16 * kThreads;
+            const unsigned gx = (unsigned)(batch < 65535 ? batch : 65535);
+            const unsigned gy = (unsigned)ceil_div(batch, 65535);
+            hipLaunchKernelGGL(gemv_quad_kernel, dim3(gx, gy), dim3(kThreads), lds, st, (int)m, (int)k, alpha, A, lda,
+                               strideA, B, strideB, beta, C, strideC, batch);
+            return launch_check("gemv_quad_kernel");
+        }
     }
     const bool narrow = (n <= 4 && m * n <= 4096);
     if (alias || narrow) {

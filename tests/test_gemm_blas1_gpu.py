@@ -50,14 +50,16 @@ def _gemm_case(oracle, m, n, kk, batch, dtype, mode, alpha=1.0, beta=0.0, seed=0
 @pytest.mark.parametrize("m,n,kk", [(1, 1, 1), (2, 2, 3), (5, 7, 3), (64, 64, 64), (65, 63, 33),
                                     (96, 1, 96), (128, 200, 70), (16, 3, 100),
                                     # register-direct path: m, n <= 64, k a multiple of 16 up to 64
-                                    (50, 33, 48), (64, 64, 16), (17, 64, 32), (64, 5, 64), (3, 64, 64)])
+                                    (50, 33, 48), (64, 64, 16), (17, 64, 32), (64, 5, 64), (3, 64, 64),
+                                    # matrix-vector products (row-quad GEMV path when m % 4 == 0)
+                                    (100, 1, 7), (1024, 1, 40), (98, 1, 5)])
 @pytest.mark.parametrize("dtype,mode", [(torch.float32, "fp32"), (torch.float32, "bf16x3"),
                                         (torch.float64, "fp64")])
 def test_gemm_matches_oracle(oracle, m, n, kk, dtype, mode):
     _gemm_case(oracle, m, n, kk, 3, dtype, mode, seed=m * 7 + n)
 
 
-@pytest.mark.parametrize("m,n,kk", [(40, 24, 17), (64, 40, 32), (30, 64, 64)])
+@pytest.mark.parametrize("m,n,kk", [(40, 24, 17), (64, 40, 32), (30, 64, 64), (64, 1, 33)])
 @pytest.mark.parametrize("dtype,mode", [(torch.float32, "fp32"), (torch.float32, "bf16x3"),
                                         (torch.float64, "fp64")])
 def test_gemm_alpha_beta(oracle, m, n, kk, dtype, mode):
