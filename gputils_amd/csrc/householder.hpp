@@ -61,6 +61,52 @@ __device__ __forceinline__ void apply_reflector_col(T *A, int64_t lda, int m, in
     if (lane == 0) x[j] -= tw;
 }
 
+// x_c <- (I - tau v v^T) x_c for the columns c = c0, c0 + 4, ... < n (the
+// columns one wavefront owns), v = [1; A[j+1:m, j]], four columns at a
+// time: their dot products are accumulated together and their reductions
+// overlap, so the wavefront waits for one set of reductions per four
+// columns instead of one per column.  Past the last column the slots work
+// on the first owned column with tau w = 0.
+template <typename T, typename ACC>
+__device__ __forceinline__ void apply_reflector_cols(T *A, int64_t lda, int m, int j, T tau, int c0, int n) {
+    const int lane = threadIdx.x & 63;
+    const T *v = A + (int64_t)j * lda;
+    for (int cq = c0; cq < n; cq += 4 * kWaves) {
+        T *x[4];
+        bool ok[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int c = cq + kWaves * q;
+            ok[q] = c < n;
+            x[q] = A + (int64_t)(ok[q] ? c : cq) * lda;
+        }
+        ACC w[4] = {ACC(0), ACC(0), ACC(0), ACC(0)};
+        for (int i = j + 1 + lane; i < m; i += kWave) {
+            const ACC vi = (ACC)v[i];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) w[q] += vi * (ACC)x[q][i];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) w[q] = wave_sum(w[q]);
+        T tw[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) tw[q] = ok[q] ? (T)((ACC)tau * (w[q] + (ACC)x[q][j])) : T(0);
+        for (int i = j + 1 + lane; i < m; i += kWave) {
+            const T vi = v[i];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (ok[q]) x[q][i] -= tw[q] * vi;
+        }
+        wave_sync();
+        if (lane == 0) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (ok[q]) x[q][j] -= tw[q];
+        }
+        wave_sync();
+    }
+}
+
 // Householder QR of the m x n (m >= n) matrix at A (LDS or global), tau[j]
 // for j < n.  s_tau: two LDS slots.  Ends with a barrier.
 template <typename T, typename ACC>
@@ -85,10 +131,7 @@ __device__ __forceinline__ void hqr_factor(T *A, int64_t lda, int m, int n, T *t
                 tau[j + 1] = t1;
             }
         }
-        if (t != T(0)) {
-            for (int c = j + 2 + ((wave - (j + 2)) & (kWaves - 1)); c < n; c += kWaves)
-                apply_reflector_col<T, ACC>(A, lda, m, j, t, c);
-        }
+        if (t != T(0)) apply_reflector_cols<T, ACC>(A, lda, m, j, t, j + 2 + ((wave - (j + 2)) & (kWaves - 1)), n);
         __syncthreads();
     }
 }
