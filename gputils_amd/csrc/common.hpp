@@ -105,6 +105,39 @@ __device__ inline T wave_sum(T v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
     return v;
 }
+
+// v + the value of the lane that DPP control CTRL selects (0 where that lane
+// lies outside the row or the row is masked off).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_add(float v) {
+    return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, true));
+}
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_add(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, ROW_MASK, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xF, true);
+    return v + __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Floating-point wave sums in registers: row shifts 1, 2, 4, 8 leave each
+// 16-lane row's sum in its lane 15, row broadcasts 15 and 31 gather the four
+// row sums in lane 63, and readlane hands that to every lane.  No LDS
+// round trips, unlike the xor shuffles.  All 64 lanes must be active.
+template <typename T>
+__device__ __forceinline__ T wave_sum_dpp(T v) {
+    v = dpp_add<0x111, 0xF>(v);  // row_shr:1
+    v = dpp_add<0x112, 0xF>(v);  // row_shr:2
+    v = dpp_add<0x114, 0xF>(v);  // row_shr:4
+    v = dpp_add<0x118, 0xF>(v);  // row_shr:8
+    v = dpp_add<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+    v = dpp_add<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+    return readlane(v, 63);
+}
+template <>
+__device__ inline float wave_sum<float>(float v) { return wave_sum_dpp(v); }
+template <>
+__device__ inline double wave_sum<double>(double v) { return wave_sum_dpp(v); }
 template <typename T>
 __device__ inline T wave_max(T v) {
 #pragma unroll
