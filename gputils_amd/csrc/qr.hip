@@ -220,6 +220,7 @@ __global__ __launch_bounds__(kThreads) void gels_ragged_kernel(const RaggedItem 
     __shared__ float tau[64];
     __shared__ float c[512];
     __shared__ float h[64];
+    __shared__ float dx[64];
     __shared__ double xd[64];
     __shared__ double rd[512];
     __shared__ double bd[512];
@@ -292,19 +293,21 @@ __global__ __launch_bounds__(kThreads) void gels_ragged_kernel(const RaggedItem 
                 if (lane == 0) h[j] = (float)(-g);
             }
             __syncthreads();
-            if (wave == 0) {
-                trsv_upper_trans<float>(S, m, n, h);                 // h = R^{-T} g
-                apply_q_vec<float, double>(S, m, m, n, tau, c, true);  // c = Q^T f
-                for (int i = lane; i < n; i += kWave) c[i] -= h[i];
-                wave_sync();
-                trsv_upper<float>(S, m, n, c);                       // c[0:n] = dx
-                for (int i = lane; i < n; i += kWave) {
-                    xd[i] += (double)c[i];
-                    c[i] = h[i];
-                }
-                wave_sync();
-                apply_q_vec<float, double>(S, m, m, n, tau, c, false); // c = dr = Q [h; d2]
+            // the two independent serial pieces of each half run on two
+            // wavefronts at once
+            if (wave == 0) trsv_upper_trans<float>(S, m, n, h);                 // h = R^{-T} g
+            if (wave == 1) apply_q_vec<float, double>(S, m, m, n, tau, c, true);  // c = Q^T f
+            __syncthreads();
+            for (int i = threadIdx.x; i < n; i += kThreads) {
+                dx[i] = c[i] - h[i];
+                c[i] = h[i];
             }
+            __syncthreads();
+            if (wave == 0) {
+                trsv_upper<float>(S, m, n, dx);                               // dx = R^{-1} (d1 - h)
+                for (int i = lane; i < n; i += kWave) xd[i] += (double)dx[i];
+            }
+            if (wave == 1) apply_q_vec<float, double>(S, m, m, n, tau, c, false);  // c = dr = Q [h; d2]
             __syncthreads();
             for (int i = threadIdx.x; i < m; i += kThreads) rd[i] += (double)c[i];
         }
