@@ -108,11 +108,14 @@ __device__ __forceinline__ void apply_reflector_cols(T *A, int64_t lda, int m, i
 }
 
 // Householder QR of the m x n (m >= n) matrix at A (LDS or global), tau[j]
-// for j < n.  s_tau: two LDS slots.  Ends with a barrier.
+// for j < n.  s_tau: two LDS slots.  Ends with a barrier.  Columns n ..
+// ntot-1 (stored after the factorised ones) receive every reflector
+// without being factorised: a right-hand side stored there ends as Q^T b.
 template <typename T, typename ACC>
-__device__ __forceinline__ void hqr_factor(T *A, int64_t lda, int m, int n, T *tau, T *s_tau) {
+__device__ __forceinline__ void hqr_factor(T *A, int64_t lda, int m, int n, T *tau, T *s_tau, int ntot = 0) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (n <= 0) return;
+    if (ntot < n) ntot = n;
     if (wave == 0) {
         const T t = make_reflector<T, ACC>(A, lda, m, 0);
         if (lane == 0) {
@@ -123,15 +126,17 @@ __device__ __forceinline__ void hqr_factor(T *A, int64_t lda, int m, int n, T *t
     __syncthreads();
     for (int j = 0; j < n; ++j) {
         const T t = s_tau[j & 1];
-        if (j + 1 < n && wave == ((j + 1) & (kWaves - 1))) {
+        if (j + 1 < ntot && wave == ((j + 1) & (kWaves - 1))) {
             apply_reflector_col<T, ACC>(A, lda, m, j, t, j + 1);
-            const T t1 = make_reflector<T, ACC>(A, lda, m, j + 1);
-            if (lane == 0) {
-                s_tau[(j + 1) & 1] = t1;
-                tau[j + 1] = t1;
+            if (j + 1 < n) {
+                const T t1 = make_reflector<T, ACC>(A, lda, m, j + 1);
+                if (lane == 0) {
+                    s_tau[(j + 1) & 1] = t1;
+                    tau[j + 1] = t1;
+                }
             }
         }
-        if (t != T(0)) apply_reflector_cols<T, ACC>(A, lda, m, j, t, j + 2 + ((wave - (j + 2)) & (kWaves - 1)), n);
+        if (t != T(0)) apply_reflector_cols<T, ACC>(A, lda, m, j, t, j + 2 + ((wave - (j + 2)) & (kWaves - 1)), ntot);
         __syncthreads();
     }
 }

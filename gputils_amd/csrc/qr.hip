@@ -171,15 +171,6 @@ __device__ __forceinline__ void trsv_upper_trans(const T *A, int64_t lda, int n,
     }
 }
 
-// One bucket of the ragged plan: a persistent grid walks the cost-sorted
-// item list with a static stride.  Per system: FP64 A -> FP32 (LDS), FP32
-// Householder QR with FP64 norms/dots, x0 = R^{-1} Q^T b, r0 = b - A x0 in
-// FP64, then `steps` refinements of the augmented system
-// [I A; A^T 0][r; x] = [b; 0] (Bjorck 1967): f = b - r - A x, g = -A^T r
-// in FP64, and with the FP32 factors h = R^{-T} g, d = Q^T f,
-// dx = R^{-1}(d[0:n] - h), dr = Q [h; d[n:m]].  This recovers FP64
-// accuracy also for inconsistent systems, where refining x alone stalls at
-// the FP32 level.  x overwrites b[0:n].
 // FP64 r_i = base_i - sum_k A[i][k] x[k] for the rows i of this thread
 // (row-major over threads, coalesced over rows); the loads of eight
 // columns are issued together so that they are in flight at once.
@@ -202,7 +193,8 @@ __device__ __forceinline__ void rows_minus_ax(const double *__restrict__ a, int 
 
 // One bucket of the ragged plan: a persistent grid walks the cost-sorted
 // item list with a static stride.  Per system: FP64 A -> FP32 (LDS), FP32
-// Householder QR with FP64 norms/dots, x0 = R^{-1} Q^T b, r0 = b - A x0 in
+// Householder QR with FP64 norms/dots (b rides along as an extra column, so
+// the factorisation leaves Q^T b), x0 = R^{-1} Q^T b, r0 = b - A x0 in
 // FP64, then `steps` refinements of the augmented system
 // [I A; A^T 0][r; x] = [b; 0] (Bjorck 1967): f = b - r - A x, g = -A^T r
 // in FP64, and with the FP32 factors h = R^{-T} g, d = Q^T f,
@@ -215,7 +207,7 @@ template <int MAXE>
 __global__ __launch_bounds__(kThreads) void gels_ragged_kernel(const RaggedItem *__restrict__ items,
                                                                int nitems, const double *__restrict__ A,
                                                                double *B, int steps) {
-    __shared__ float S[MAXE];
+    __shared__ float S[MAXE + 512];  // A (m x n, m*n <= MAXE), then b (m <= 512)
     __shared__ float s_tau[2];
     __shared__ float tau[64];
     __shared__ float c[512];
@@ -243,19 +235,17 @@ __global__ __launch_bounds__(kThreads) void gels_ragged_kernel(const RaggedItem 
             }
             for (; e0 < mn; e0 += kThreads) S[e0] = (float)a[e0];
         }
+        float *qb = S + m * n;  // b as column n: the factorisation turns it into Q^T b
         for (int i = threadIdx.x; i < m; i += kThreads) {
             const double v = bb[i];
             bd[i] = v;
-            c[i] = (float)v;
+            qb[i] = (float)v;
         }
         __syncthreads();
-        hqr_factor<float, double>(S, m, m, n, tau, s_tau);
-        if (wave == 0) {
-            apply_q_vec<float, double>(S, m, m, n, tau, c, true);
-            trsv_upper<float>(S, m, n, c);
-        }
+        hqr_factor<float, double>(S, m, m, n, tau, s_tau, n + 1);
+        if (wave == 0) trsv_upper<float>(S, m, n, qb);  // x0 = R^{-1} (Q^T b)[0:n]
         __syncthreads();
-        for (int i = threadIdx.x; i < n; i += kThreads) xd[i] = (double)c[i];
+        for (int i = threadIdx.x; i < n; i += kThreads) xd[i] = (double)qb[i];
         __syncthreads();
         // r0 = b - A x0 (FP64, A from global memory)
         rows_minus_ax(a, m, n, xd, bd, nullptr, rd);
