@@ -287,6 +287,130 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MODE =
         }
 }
 
+// Staged kernel for m, n <= 64 and k in {16, 32, 48, 64} (FP32) with
+// 16-byte aligned columns (m % 4 == 0 and ld % 4 == 0): one workgroup per
+// matrix reads A_i and B_i with 16-byte loads, consecutive threads on
+// consecutive addresses (each column one contiguous read, all eight loads
+// of a thread in flight at once), into LDS tiles with row stride 65 (bank
+// conflict free for the operand reads below); after one barrier every
+// wave computes its 32x32 quadrant of C as in gemm_reg_kernel (k = 16g + t
+// for lane group g at step t) and stores it from the accumulators.
+constexpr int kStS = 65;  // LDS row stride of the staged tiles (floats)
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void gemm_staged_kernel(int m, int n, int k, float alpha,
+                                                               const float *__restrict__ A, int64_t lda,
+                                                               int64_t strideA, const float *__restrict__ B,
+                                                               int64_t ldb, int64_t strideB, float beta, float *C,
+                                                               int64_t ldc, int64_t strideC, int vec_c,
+                                                               int64_t batch) {
+    __shared__ float As[64 * kStS];  // As[j * kStS + i] = A(i, j), zero outside m x k
+    __shared__ float Bs[64 * kStS];  // Bs[j * kStS + p] = B(p, j), zero outside k x n
+    const int64_t bidx = blockIdx.x + (int64_t)blockIdx.y * 65535;
+    if (bidx >= batch) return;  // uniform per workgroup
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const float *a = A + bidx * strideA;
+    const float *b = B + bidx * strideB;
+    float *cm = C + bidx * strideC;
+    {
+        f32x4 va[4], vb[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = t + kThreads * u, j = e >> 4, q4 = 4 * (e & 15);
+            const bool ina = j < k && q4 < m, inb = j < n && q4 < k;
+            va[u] = *reinterpret_cast<const f32x4 *>(a + (int64_t)(ina ? j : 0) * lda + (ina ? q4 : 0));
+            vb[u] = *reinterpret_cast<const f32x4 *>(b + (int64_t)(inb ? j : 0) * ldb + (inb ? q4 : 0));
+            if (!ina) va[u] = f32x4{0.f, 0.f, 0.f, 0.f};
+            if (!inb) vb[u] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = t + kThreads * u, j = e >> 4, q4 = 4 * (e & 15);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                As[j * kStS + q4 + r] = va[u][r];
+                Bs[j * kStS + q4 + r] = vb[u][r];
+            }
+        }
+    }
+    __syncthreads();
+    const int g = lane >> 4, c = lane & 15;
+    const int wm = 32 * (wave & 1), wn = 32 * (wave >> 1);
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] = f32x4{0.f, 0.f, 0.f, 0.f};
+    if constexpr (MODE == GPUTILS_GEMM_BF16X3) {
+#pragma unroll
+        for (int s8 = 0; s8 < 2; ++s8) {
+            bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+            for (int x = 0; x < 2; ++x) {
+                float av[8], bv[8];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int kk = 16 * g + 8 * s8 + e;
+                    av[e] = As[kk * kStS + wm + 16 * x + c];
+                    bv[e] = Bs[(wn + 16 * x + c) * kStS + kk];
+                }
+                split_bf16(av, ah[x], al[x]);
+                split_bf16(bv, bh[x], bl[x]);
+            }
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y) {
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[x], bh[y], acc[x][y], 0, 0, 0);
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bl[y], acc[x][y], 0, 0, 0);
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[x], bh[y], acc[x][y], 0, 0, 0);
+                }
+        }
+    } else {
+#pragma unroll
+        for (int tt = 0; tt < 16; ++tt) {
+            const int kk = 16 * g + tt;
+            float av[2], bv[2];
+#pragma unroll
+            for (int x = 0; x < 2; ++x) {
+                av[x] = As[kk * kStS + wm + 16 * x + c];
+                bv[x] = Bs[(wn + 16 * x + c) * kStS + kk];
+            }
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[x], bv[y], acc[x][y], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            const int row0 = wm + 16 * x + 4 * g, col = wn + 16 * y + c;
+            if (col >= n) continue;
+            float *dst = cm + row0 + (int64_t)col * ldc;
+            if (vec_c && row0 + 3 < m) {
+                f32x4 v = {alpha * acc[x][y][0], alpha * acc[x][y][1], alpha * acc[x][y][2], alpha * acc[x][y][3]};
+                if (beta != 0.f) {
+                    const f32x4 o = *reinterpret_cast<const f32x4 *>(dst);
+                    v[0] += beta * o[0];
+                    v[1] += beta * o[1];
+                    v[2] += beta * o[2];
+                    v[3] += beta * o[3];
+                }
+                *reinterpret_cast<f32x4 *>(dst) = v;
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    if (row0 + r >= m) continue;
+                    float v = alpha * acc[x][y][r];
+                    if (beta != 0.f) v += beta * dst[r];
+                    dst[r] = v;
+                }
+            }
+        }
+}
+
 // Batched GEMV y_i <- alpha A_i x_i + beta y_i (FP32, m % 4 == 0,
 // m / 4 <= kThreads, 16-byte aligned columns), y may alias x (the in-place
 // Nullspace::project, tensor.cuh:2084).  One workgroup per matrix: x is
@@ -455,6 +579,14 @@ int run(int64_t m, int64_t n, int64_t k, T alpha, const T *A, int64_t lda, int64
             const int vec_c = (reinterpret_cast<uintptr_t>(C) % 16 == 0) && ldc % 4 == 0 && strideC % 4 == 0;
             const unsigned gx = (unsigned)(batch < 65535 ? batch : 65535);
             const unsigned gy = (unsigned)ceil_div(batch, 65535);
+            const bool a_vec = (reinterpret_cast<uintptr_t>(A) % 16 == 0) && lda % 4 == 0 && strideA % 4 == 0 &&
+                               m % 4 == 0;
+            if (a_vec) {
+                hipLaunchKernelGGL((gemm_staged_kernel<MODE>), dim3(gx, gy), dim3(kThreads), 0, st, (int)m, (int)n,
+                                   (int)k, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc, strideC, vec_c,
+                                   batch);
+                return launch_check("gemm_staged_kernel");
+            }
             hipLaunchKernelGGL((gemm_reg_kernel<MODE>), dim3(gx, gy), dim3(kThreads), 0, st, (int)m, (int)n,
                                (int)k, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc, strideC, vec_c,
                                batch);
