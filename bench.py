@@ -307,7 +307,7 @@ def bench_gemm(a):
     if not a.no_cpu_baseline:
         cpu = _cpu_gemm(A, B)
     m = modes["fp32"]
-    kern = "gemm_tiled_kernel<float,0>"
+    kern = "gemm_staged_kernel<0>"
     return _line("batched GEMMs/s (C = A B, 64x64x64, FP32)", k / (m["ms_per_step"] * 1e-3), "GEMMs/s", a,
                  m["ms_per_step"], "f32",
                  {"workload": "gemm: k=16384 64x64 by 64x64 FP32, U[-1,1]", "k": k, "m": n, "n": n, "kk": n},
