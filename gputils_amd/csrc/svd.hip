@@ -138,7 +138,7 @@ __global__ __launch_bounds__(kThreads) void gesvd_kernel(int m, int n, const T *
     // 2. one-sided Jacobi on W, V accumulated in X
     const int n2 = n + (n & 1), n1 = n2 - 1, npairs = n2 >> 1;
     const int grp = tid >> 3, gl = tid & 7;
-    const double tol = (double)(n > 2 ? n : 2) * DBL_EPSILON;
+    const double tol = (double)(n > 2 ? n : 2) * DBL_EPSILON, tol2 = tol * tol;
     int sweeps = 0;
     bool converged = (n1 == 0);
     while (!converged && sweeps < kMaxSweeps) {
@@ -160,12 +160,21 @@ __global__ __launch_bounds__(kThreads) void gesvd_kernel(int m, int n, const T *
                     al = group_sum(al, 8);
                     be = group_sum(be, 8);
                     ga = group_sum(ga, 8);
-                    if (fabs(ga) > tol * sqrt(al) * sqrt(be)) {
-                        const double zeta = (be - al) / (2.0 * ga);
-                        const double az = fabs(zeta);
-                        const double t = (az > 1e150) ? 0.5 / zeta
-                                                      : (zeta >= 0.0 ? 1.0 : -1.0) / (az + sqrt(1.0 + zeta * zeta));
-                        const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+                    // |ga| > tol sqrt(al be), squared: no square roots on the
+                    // round's critical path
+                    if (ga * ga > tol2 * al * be) {
+                        // t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)) with
+                        // zeta = (be - al) / (2 ga), multiplied through by
+                        // 2 |ga|: one square root and one division
+                        const double d = be - al, g2 = 2.0 * fabs(ga);
+                        const double sg = (d == 0.0 || ((d > 0.0) == (ga > 0.0))) ? 1.0 : -1.0;
+                        const double t = sg * g2 / (fabs(d) + sqrt(fma(d, d, g2 * g2)));
+                        // cs = 1 / sqrt(1 + t^2): hardware estimate and one
+                        // Newton step (relative error ~ the estimate's squared)
+                        const double t2p1 = fma(t, t, 1.0);
+                        double cs = __builtin_amdgcn_rsq(t2p1);
+                        cs = cs * fma(-0.5 * t2p1, cs * cs, 1.5);
+                        const double sn = cs * t;
                         for (int i = gl; i < n; i += 8) {
                             const double x = wp[i], y = wq[i];
                             wp[i] = cs * x - sn * y;
