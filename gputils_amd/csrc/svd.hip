@@ -77,9 +77,26 @@ __host__ __device__ inline Layout layout(int m, int n, bool resident) {
     return L;
 }
 
-__device__ __forceinline__ double group_sum(double x, int width) {
-    for (int off = 1; off < width; off <<= 1) x += __shfl_xor(x, off, kWave);
-    return x;
+// x + (x of the lane that DPP control CTRL selects), within 16-lane rows
+template <int CTRL>
+__device__ __forceinline__ double dpp_plus(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return x + __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sums over aligned groups of 8 / 16 lanes, result in every lane of the
+// group: quad permutations [1,0,3,2] and [2,3,0,1] sum each quad, the row
+// half mirror (lane i <-> 7 - i) adds the other quad of the 8, the row
+// mirror (lane i <-> 15 - i) the other 8 of the 16.  Register moves only.
+__device__ __forceinline__ double group8_sum(double x) {
+    x = dpp_plus<0xB1>(x);      // quad_perm [1,0,3,2]
+    x = dpp_plus<0x4E>(x);      // quad_perm [2,3,0,1]
+    return dpp_plus<0x141>(x);  // row_half_mirror
+}
+__device__ __forceinline__ double group16_sum(double x) {
+    return dpp_plus<0x140>(group8_sum(x));  // row_mirror
 }
 
 // Pair i of round r in a round-robin tournament over n2 (even) players:
@@ -157,9 +174,9 @@ __global__ __launch_bounds__(kThreads) void gesvd_kernel(int m, int n, const T *
                         be = fma(y, y, be);
                         ga = fma(x, y, ga);
                     }
-                    al = group_sum(al, 8);
-                    be = group_sum(be, 8);
-                    ga = group_sum(ga, 8);
+                    al = group8_sum(al);
+                    be = group8_sum(be);
+                    ga = group8_sum(ga);
                     // |ga| > tol sqrt(al be), squared: no square roots on the
                     // round's critical path
                     if (ga * ga > tol2 * al * be) {
@@ -200,7 +217,7 @@ __global__ __launch_bounds__(kThreads) void gesvd_kernel(int m, int n, const T *
     for (int j = grp; j < n; j += kThreads / 8) {
         double s = 0.0;
         for (int i = gl; i < n; i += 8) s = fma(W[i + j * n], W[i + j * n], s);
-        s = group_sum(s, 8);
+        s = group8_sum(s);
         if (gl == 0) sig[j] = sqrt(s);
     }
     __syncthreads();
@@ -302,7 +319,7 @@ __global__ __launch_bounds__(kThreads) void gesvd_kernel(int m, int n, const T *
                 const double *v = G + (int64_t)j * m;
                 double w = 0.0;
                 for (int i = j + 1 + l16; i < m; i += 16) w = fma(v[i], xc[i], w);
-                w = group_sum(w, 16) + xc[j];
+                w = group16_sum(w) + xc[j];
                 const double tw = t * w;
                 for (int i = j + 1 + l16; i < m; i += 16) xc[i] -= tw * v[i];
                 wave_sync();
