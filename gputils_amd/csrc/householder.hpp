@@ -45,6 +45,62 @@ __device__ __forceinline__ T make_reflector(T *A, int64_t lda, int m, int j) {
     return (T)tau;
 }
 
+// The lookahead step of the owner of column j + 1, fused: reflector j
+// (tau) is applied to column j + 1 and reflector j + 1 is formed from the
+// result while the column stays in registers (rows j + 1 + lane + 64 u,
+// u < R, so m - j - 1 <= 64 R), reading the column and v once and writing
+// the column once.  The arithmetic is apply_reflector_col followed by
+// make_reflector.  Returns tau_{j+1}.
+template <typename T, typename ACC, int R>
+__device__ __forceinline__ T update_and_reflect(T *A, int64_t lda, int m, int j, T tau) {
+    const int lane = threadIdx.x & 63;
+    const T *v = A + (int64_t)j * lda;
+    T *x = A + (int64_t)(j + 1) * lda;
+    T vr[R], xr[R];
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+        const int i = j + 1 + lane + kWave * u;
+        const int ic = i < m ? i : m - 1;  // clamped: every load is valid
+        vr[u] = v[ic];
+        xr[u] = x[ic];
+        if (i >= m) vr[u] = xr[u] = T(0);
+    }
+    const T xj = x[j];
+    ACC w = ACC(0);
+#pragma unroll
+    for (int u = 0; u < R; ++u) w += (ACC)vr[u] * (ACC)xr[u];
+    w = wave_sum(w) + (ACC)xj;
+    const T tw = (T)((ACC)tau * w);
+#pragma unroll
+    for (int u = 0; u < R; ++u) xr[u] -= tw * vr[u];
+    // reflector j + 1: alpha = row j + 1 (lane 0, u = 0), x = rows j + 2 ..
+    ACC sq = ACC(0);
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+        const ACC e = (ACC)xr[u];
+        sq += (u == 0 && lane == 0) ? ACC(0) : e * e;
+    }
+    sq = wave_sum(sq);
+    const ACC alpha = (ACC)readlane(xr[0], 0);
+    T t1 = T(0);
+    if (sq != ACC(0)) {  // else H = I (dlarfg with x = 0): the column stays as it is
+        const ACC norm = sqrt(alpha * alpha + sq);
+        const ACC beta = (alpha >= ACC(0)) ? -norm : norm;
+        t1 = (T)((beta - alpha) / beta);
+        const T scale = (T)(ACC(1) / (alpha - beta));
+#pragma unroll
+        for (int u = 0; u < R; ++u) xr[u] = (u == 0 && lane == 0) ? (T)beta : xr[u] * scale;
+    }
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+        const int i = j + 1 + lane + kWave * u;
+        if (i < m) x[i] = xr[u];
+    }
+    if (lane == 0) x[j] = xj - tw;
+    wave_sync();
+    return t1;
+}
+
 // x <- (I - tau v v^T) x on rows j..m-1 of column c, v = [1; A[j+1:m, j]]
 // (one wavefront).
 template <typename T, typename ACC>
@@ -127,12 +183,20 @@ __device__ __forceinline__ void hqr_factor(T *A, int64_t lda, int m, int n, T *t
     for (int j = 0; j < n; ++j) {
         const T t = s_tau[j & 1];
         if (j + 1 < ntot && wave == ((j + 1) & (kWaves - 1))) {
-            apply_reflector_col<T, ACC>(A, lda, m, j, t, j + 1);
-            if (j + 1 < n) {
-                const T t1 = make_reflector<T, ACC>(A, lda, m, j + 1);
+            if (j + 1 < n && m <= 8 * kWave) {
+                const T t1 = update_and_reflect<T, ACC, 8>(A, lda, m, j, t);
                 if (lane == 0) {
                     s_tau[(j + 1) & 1] = t1;
                     tau[j + 1] = t1;
+                }
+            } else {
+                apply_reflector_col<T, ACC>(A, lda, m, j, t, j + 1);
+                if (j + 1 < n) {
+                    const T t1 = make_reflector<T, ACC>(A, lda, m, j + 1);
+                    if (lane == 0) {
+                        s_tau[(j + 1) & 1] = t1;
+                        tau[j + 1] = t1;
+                    }
                 }
             }
         }
