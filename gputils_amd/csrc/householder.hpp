@@ -51,7 +51,7 @@ __device__ __forceinline__ T make_reflector(T *A, int64_t lda, int m, int j) {
 // u < R, so m - j - 1 <= 64 R), reading the column and v once and writing
 // the column once.  The arithmetic is apply_reflector_col followed by
 // make_reflector.  Returns tau_{j+1}.
-template <typename T, typename ACC, int R>
+template <typename T, typename ACC, typename DOT, int R>
 __device__ __forceinline__ T update_and_reflect(T *A, int64_t lda, int m, int j, T tau) {
     const int lane = threadIdx.x & 63;
     const T *v = A + (int64_t)j * lda;
@@ -66,11 +66,11 @@ __device__ __forceinline__ T update_and_reflect(T *A, int64_t lda, int m, int j,
         if (i >= m) vr[u] = xr[u] = T(0);
     }
     const T xj = x[j];
-    ACC w = ACC(0);
+    DOT w = DOT(0);
 #pragma unroll
-    for (int u = 0; u < R; ++u) w += (ACC)vr[u] * (ACC)xr[u];
-    w = wave_sum(w) + (ACC)xj;
-    const T tw = (T)((ACC)tau * w);
+    for (int u = 0; u < R; ++u) w += (DOT)vr[u] * (DOT)xr[u];
+    w = wave_sum(w) + (DOT)xj;
+    const T tw = (T)((DOT)tau * w);
 #pragma unroll
     for (int u = 0; u < R; ++u) xr[u] -= tw * vr[u];
     // reflector j + 1: alpha = row j + 1 (lane 0, u = 0), x = rows j + 2 ..
@@ -103,15 +103,15 @@ __device__ __forceinline__ T update_and_reflect(T *A, int64_t lda, int m, int j,
 
 // x <- (I - tau v v^T) x on rows j..m-1 of column c, v = [1; A[j+1:m, j]]
 // (one wavefront).
-template <typename T, typename ACC>
+template <typename T, typename DOT>
 __device__ __forceinline__ void apply_reflector_col(T *A, int64_t lda, int m, int j, T tau, int c) {
     const int lane = threadIdx.x & 63;
     const T *v = A + (int64_t)j * lda;
     T *x = A + (int64_t)c * lda;
-    ACC w = ACC(0);
-    for (int i = j + 1 + lane; i < m; i += kWave) w += (ACC)v[i] * (ACC)x[i];
-    w = wave_sum(w) + (ACC)x[j];
-    const T tw = (T)((ACC)tau * w);
+    DOT w = DOT(0);
+    for (int i = j + 1 + lane; i < m; i += kWave) w += (DOT)v[i] * (DOT)x[i];
+    w = wave_sum(w) + (DOT)x[j];
+    const T tw = (T)((DOT)tau * w);
     for (int i = j + 1 + lane; i < m; i += kWave) x[i] -= tw * v[i];
     wave_sync();
     if (lane == 0) x[j] -= tw;
@@ -123,7 +123,7 @@ __device__ __forceinline__ void apply_reflector_col(T *A, int64_t lda, int m, in
 // overlap, so the wavefront waits for one set of reductions per four
 // columns instead of one per column.  Past the last column the slots work
 // on the first owned column with tau w = 0.
-template <typename T, typename ACC>
+template <typename T, typename DOT>
 __device__ __forceinline__ void apply_reflector_cols(T *A, int64_t lda, int m, int j, T tau, int c0, int n) {
     const int lane = threadIdx.x & 63;
     const T *v = A + (int64_t)j * lda;
@@ -136,17 +136,17 @@ __device__ __forceinline__ void apply_reflector_cols(T *A, int64_t lda, int m, i
             ok[q] = c < n;
             x[q] = A + (int64_t)(ok[q] ? c : cq) * lda;
         }
-        ACC w[4] = {ACC(0), ACC(0), ACC(0), ACC(0)};
+        DOT w[4] = {DOT(0), DOT(0), DOT(0), DOT(0)};
         for (int i = j + 1 + lane; i < m; i += kWave) {
-            const ACC vi = (ACC)v[i];
+            const DOT vi = (DOT)v[i];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) w[q] += vi * (ACC)x[q][i];
+            for (int q = 0; q < 4; ++q) w[q] += vi * (DOT)x[q][i];
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) w[q] = wave_sum(w[q]);
         T tw[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) tw[q] = ok[q] ? (T)((ACC)tau * (w[q] + (ACC)x[q][j])) : T(0);
+        for (int q = 0; q < 4; ++q) tw[q] = ok[q] ? (T)((DOT)tau * (w[q] + (DOT)x[q][j])) : T(0);
         for (int i = j + 1 + lane; i < m; i += kWave) {
             const T vi = v[i];
 #pragma unroll
@@ -164,10 +164,11 @@ __device__ __forceinline__ void apply_reflector_cols(T *A, int64_t lda, int m, i
 }
 
 // Householder QR of the m x n (m >= n) matrix at A (LDS or global), tau[j]
-// for j < n.  s_tau: two LDS slots.  Ends with a barrier.  Columns n ..
+// for j < n, column norms accumulated in ACC and the reflectors' dot
+// products in DOT.  s_tau: two LDS slots.  Ends with a barrier.  Columns n ..
 // ntot-1 (stored after the factorised ones) receive every reflector
 // without being factorised: a right-hand side stored there ends as Q^T b.
-template <typename T, typename ACC>
+template <typename T, typename ACC, typename DOT = ACC>
 __device__ __forceinline__ void hqr_factor(T *A, int64_t lda, int m, int n, T *tau, T *s_tau, int ntot = 0) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (n <= 0) return;
@@ -184,13 +185,13 @@ __device__ __forceinline__ void hqr_factor(T *A, int64_t lda, int m, int n, T *t
         const T t = s_tau[j & 1];
         if (j + 1 < ntot && wave == ((j + 1) & (kWaves - 1))) {
             if (j + 1 < n && m <= 8 * kWave) {
-                const T t1 = update_and_reflect<T, ACC, 8>(A, lda, m, j, t);
+                const T t1 = update_and_reflect<T, ACC, DOT, 8>(A, lda, m, j, t);
                 if (lane == 0) {
                     s_tau[(j + 1) & 1] = t1;
                     tau[j + 1] = t1;
                 }
             } else {
-                apply_reflector_col<T, ACC>(A, lda, m, j, t, j + 1);
+                apply_reflector_col<T, DOT>(A, lda, m, j, t, j + 1);
                 if (j + 1 < n) {
                     const T t1 = make_reflector<T, ACC>(A, lda, m, j + 1);
                     if (lane == 0) {
@@ -200,7 +201,7 @@ __device__ __forceinline__ void hqr_factor(T *A, int64_t lda, int m, int n, T *t
                 }
             }
         }
-        if (t != T(0)) apply_reflector_cols<T, ACC>(A, lda, m, j, t, j + 2 + ((wave - (j + 2)) & (kWaves - 1)), ntot);
+        if (t != T(0)) apply_reflector_cols<T, DOT>(A, lda, m, j, t, j + 2 + ((wave - (j + 2)) & (kWaves - 1)), ntot);
         __syncthreads();
     }
 }

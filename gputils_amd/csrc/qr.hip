@@ -193,7 +193,7 @@ __device__ __forceinline__ void rows_minus_ax(const double *__restrict__ a, int 
 
 // One bucket of the ragged plan: a persistent grid walks the cost-sorted
 // item list with a static stride.  Per system: FP64 A -> FP32 (LDS), FP32
-// Householder QR with FP64 norms/dots (b rides along as an extra column, so
+// Householder QR with FP64 column norms (b rides along as an extra column, so
 // the factorisation leaves Q^T b), x0 = R^{-1} Q^T b, r0 = b - A x0 in
 // FP64, then `steps` refinements of the augmented system
 // [I A; A^T 0][r; x] = [b; 0] (Bjorck 1967): f = b - r - A x, g = -A^T r
@@ -242,7 +242,7 @@ __global__ __launch_bounds__(kThreads) void gels_ragged_kernel(const RaggedItem 
             qb[i] = (float)v;
         }
         __syncthreads();
-        hqr_factor<float, double>(S, m, m, n, tau, s_tau, n + 1);
+        hqr_factor<float, double, float>(S, m, m, n, tau, s_tau, n + 1);  // FP64 norms, FP32 dots
         if (wave == 0) trsv_upper<float>(S, m, n, qb);  // x0 = R^{-1} (Q^T b)[0:n]
         __syncthreads();
         for (int i = threadIdx.x; i < n; i += kThreads) xd[i] = (double)qb[i];
@@ -286,7 +286,7 @@ __global__ __launch_bounds__(kThreads) void gels_ragged_kernel(const RaggedItem 
             // the two independent serial pieces of each half run on two
             // wavefronts at once
             if (wave == 0) trsv_upper_trans<float>(S, m, n, h);                 // h = R^{-T} g
-            if (wave == 1) apply_q_vec<float, double>(S, m, m, n, tau, c, true);  // c = Q^T f
+            if (wave == 1) apply_q_vec<float, float>(S, m, m, n, tau, c, true);  // c = Q^T f
             __syncthreads();
             for (int i = threadIdx.x; i < n; i += kThreads) {
                 dx[i] = c[i] - h[i];
@@ -297,7 +297,7 @@ __global__ __launch_bounds__(kThreads) void gels_ragged_kernel(const RaggedItem 
                 trsv_upper<float>(S, m, n, dx);                               // dx = R^{-1} (d1 - h)
                 for (int i = lane; i < n; i += kWave) xd[i] += (double)dx[i];
             }
-            if (wave == 1) apply_q_vec<float, double>(S, m, m, n, tau, c, false);  // c = dr = Q [h; d2]
+            if (wave == 1) apply_q_vec<float, float>(S, m, m, n, tau, c, false);  // c = dr = Q [h; d2]
             __syncthreads();
             for (int i = threadIdx.x; i < m; i += kThreads) rd[i] += (double)c[i];
         }
