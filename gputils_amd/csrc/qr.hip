@@ -16,6 +16,8 @@
 // tau = (beta - alpha)/beta, v = x / (alpha - beta)), so one barrier per
 // column suffices.
 #include <algorithm>
+#include <queue>
+#include <utility>
 #include <vector>
 
 #include "common.hpp"
@@ -191,8 +193,9 @@ __device__ __forceinline__ void rows_minus_ax(const double *__restrict__ a, int 
     }
 }
 
-// One bucket of the ragged plan: a persistent grid walks the cost-sorted
-// item list with a static stride.  Per system: FP64 A -> FP32 (LDS), FP32
+// One bucket of the ragged plan: workgroup w solves the items
+// wstart[w] .. wstart[w+1]-1, a range the plan balanced by estimated cost
+// (longest-processing-time assignment).  Per system: FP64 A -> FP32 (LDS), FP32
 // Householder QR with FP64 column norms (b rides along as an extra column, so
 // the factorisation leaves Q^T b), x0 = R^{-1} Q^T b, r0 = b - A x0 in
 // FP64, then `steps` refinements of the augmented system
@@ -205,8 +208,9 @@ __device__ __forceinline__ void rows_minus_ax(const double *__restrict__ a, int 
 // to hide their latency.
 template <int MAXE>
 __global__ __launch_bounds__(kThreads) void gels_ragged_kernel(const RaggedItem *__restrict__ items,
-                                                               int nitems, const double *__restrict__ A,
-                                                               double *B, int steps) {
+                                                               const int *__restrict__ wstart,
+                                                               const double *__restrict__ A, double *B,
+                                                               int steps) {
     __shared__ float S[MAXE + 512];  // A (m x n, m*n <= MAXE), then b (m <= 512)
     __shared__ float s_tau[2];
     __shared__ float tau[64];
@@ -217,7 +221,8 @@ __global__ __launch_bounds__(kThreads) void gels_ragged_kernel(const RaggedItem 
     __shared__ double rd[512];
     __shared__ double bd[512];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const int it_end = wstart[blockIdx.x + 1];
+    for (int it = wstart[blockIdx.x]; it < it_end; ++it) {
         const RaggedItem w = items[it];
         const int m = w.m, n = w.n;
         const double *a = A + w.offA;
@@ -353,10 +358,12 @@ using namespace gputils;
 struct gputils_gels_plan {
     int64_t count = 0;
     qr::RaggedItem *d_items = nullptr;
+    int *d_wstart = nullptr;  // per bucket: grid + 1 item offsets, one range per workgroup
     struct Bucket {
         int maxe;
         int begin, count;
         int grid;
+        int wbase;  // first entry of this bucket's offsets in d_wstart
     };
     std::vector<Bucket> buckets;
 };
@@ -472,10 +479,13 @@ int gputils_gels_ragged_plan_create(gputils_gels_plan_t *plan, int64_t count, co
         while (bk + 1 < kNb && e > kBucketMax[bk]) ++bk;
         return bk;
     };
+    // estimated cost of one system: the factorisation's m n^2 work spread
+    // over a wavefront plus the n-step serial chains of the solves
+    auto cost_of = [&](int i) { return (double)m[i] * n[i] * n[i] / 64.0 + 2000.0 * n[i]; };
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
         const int bx = bucket_of(x), by = bucket_of(y);
         if (bx != by) return bx > by;
-        return (int64_t)m[x] * n[x] * n[x] > (int64_t)m[y] * n[y] * n[y];
+        return cost_of(x) > cost_of(y);
     });
     auto *p = new gputils_gels_plan();
     p->count = count;
@@ -483,32 +493,59 @@ int gputils_gels_ragged_plan_create(gputils_gels_plan_t *plan, int64_t count, co
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
         cus = prop.multiProcessorCount;
+    // per bucket: workgroups resident at once (the kernel's LDS is the
+    // bucket's FP32 matrix and b column plus 13576 bytes of per-system
+    // vectors), then a longest-processing-time assignment of the
+    // cost-sorted items to them; each workgroup's items are stored
+    // contiguously and wstart holds the range boundaries
+    std::vector<int> sorted_items;
+    std::vector<int> wstart;
+    sorted_items.reserve((size_t)count);
     int start = 0;
     for (int bk = kNb - 1; bk >= 0; --bk) {
         int cnt = 0;
         while (start + cnt < count && bucket_of(order[start + cnt]) == bk) ++cnt;
         if (cnt > 0) {
-            const int lds = kBucketMax[bk] * 4 + 12 * 1024;  // S plus the per-system vectors
+            const int lds = (kBucketMax[bk] + 512) * 4 + 13576;
             const int per_cu = std::max(1, std::min(8, 160 * 1024 / lds));
-            p->buckets.push_back({kBucketMax[bk], start, cnt, std::min(cnt, cus * per_cu)});
+            const int grid = std::min(cnt, cus * per_cu);
+            std::vector<std::vector<int>> bins((size_t)grid);
+            std::priority_queue<std::pair<double, int>, std::vector<std::pair<double, int>>,
+                                std::greater<std::pair<double, int>>> heap;
+            for (int w = 0; w < grid; ++w) heap.push({0.0, w});
+            for (int k = 0; k < cnt; ++k) {
+                const int o = order[start + k];
+                auto top = heap.top();
+                heap.pop();
+                bins[(size_t)top.second].push_back(o);
+                heap.push({top.first + cost_of(o), top.second});
+            }
+            const int begin = (int)sorted_items.size();
+            p->buckets.push_back({kBucketMax[bk], begin, cnt, grid, (int)wstart.size()});
+            for (int w = 0; w < grid; ++w) {
+                wstart.push_back((int)sorted_items.size());
+                for (int o : bins[(size_t)w]) sorted_items.push_back(o);
+            }
+            wstart.push_back((int)sorted_items.size());
         }
         start += cnt;
     }
     for (int64_t i = 0; i < count; ++i) {
-        const int o = order[i];
+        const int o = sorted_items[(size_t)i];
         items[i] = {m[o], n[o], offA[o], offB[o]};
     }
     if (count > 0) {
         hipError_t e = hipMalloc(&p->d_items, sizeof(qr::RaggedItem) * (size_t)count);
+        if (e == hipSuccess) e = hipMalloc(&p->d_wstart, sizeof(int) * wstart.size());
+        if (e == hipSuccess)
+            e = hipMemcpy(p->d_items, items.data(), sizeof(qr::RaggedItem) * (size_t)count, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(p->d_wstart, wstart.data(), sizeof(int) * wstart.size(), hipMemcpyHostToDevice);
         if (e != hipSuccess) {
+            if (p->d_items) (void)hipFree(p->d_items);
+            if (p->d_wstart) (void)hipFree(p->d_wstart);
             delete p;
-            return hip_fail(e, "hipMalloc(plan items)");
-        }
-        e = hipMemcpy(p->d_items, items.data(), sizeof(qr::RaggedItem) * (size_t)count, hipMemcpyHostToDevice);
-        if (e != hipSuccess) {
-            (void)hipFree(p->d_items);
-            delete p;
-            return hip_fail(e, "hipMemcpy(plan items)");
+            return hip_fail(e, "plan upload");
         }
     }
     *plan = p;
@@ -518,6 +555,7 @@ int gputils_gels_ragged_plan_create(gputils_gels_plan_t *plan, int64_t count, co
 int gputils_gels_ragged_plan_destroy(gputils_gels_plan_t plan) {
     if (plan == nullptr) return GPUTILS_OK;
     if (plan->d_items) GPUTILS_HIP(hipFree(plan->d_items));
+    if (plan->d_wstart) GPUTILS_HIP(hipFree(plan->d_wstart));
     delete plan;
     return GPUTILS_OK;
 }
@@ -531,17 +569,18 @@ int gputils_dgels_ragged(gputils_gels_plan_t plan, const double *A, double *B, i
     GPUTILS_ARG(B != nullptr, 3, "B is NULL");
     hipStream_t st = as_stream(stream);
     for (const auto &bk : plan->buckets) {
-        const qr::RaggedItem *it = plan->d_items + bk.begin;
+        const qr::RaggedItem *it = plan->d_items;  // wstart holds absolute item offsets
+        const int *ws = plan->d_wstart + bk.wbase;
         if (bk.maxe == 2048)
-            hipLaunchKernelGGL((qr::gels_ragged_kernel<2048>), dim3(bk.grid), dim3(qr::kThreads), 0, st, it, bk.count, A, B, refine_steps);
+            hipLaunchKernelGGL((qr::gels_ragged_kernel<2048>), dim3(bk.grid), dim3(qr::kThreads), 0, st, it, ws, A, B, refine_steps);
         else if (bk.maxe == 4096)
-            hipLaunchKernelGGL((qr::gels_ragged_kernel<4096>), dim3(bk.grid), dim3(qr::kThreads), 0, st, it, bk.count, A, B, refine_steps);
+            hipLaunchKernelGGL((qr::gels_ragged_kernel<4096>), dim3(bk.grid), dim3(qr::kThreads), 0, st, it, ws, A, B, refine_steps);
         else if (bk.maxe == 8192)
-            hipLaunchKernelGGL((qr::gels_ragged_kernel<8192>), dim3(bk.grid), dim3(qr::kThreads), 0, st, it, bk.count, A, B, refine_steps);
+            hipLaunchKernelGGL((qr::gels_ragged_kernel<8192>), dim3(bk.grid), dim3(qr::kThreads), 0, st, it, ws, A, B, refine_steps);
         else if (bk.maxe == 16384)
-            hipLaunchKernelGGL((qr::gels_ragged_kernel<16384>), dim3(bk.grid), dim3(qr::kThreads), 0, st, it, bk.count, A, B, refine_steps);
+            hipLaunchKernelGGL((qr::gels_ragged_kernel<16384>), dim3(bk.grid), dim3(qr::kThreads), 0, st, it, ws, A, B, refine_steps);
         else
-            hipLaunchKernelGGL((qr::gels_ragged_kernel<32768>), dim3(bk.grid), dim3(qr::kThreads), 0, st, it, bk.count, A, B, refine_steps);
+            hipLaunchKernelGGL((qr::gels_ragged_kernel<32768>), dim3(bk.grid), dim3(qr::kThreads), 0, st, it, ws, A, B, refine_steps);
         int rc = launch_check("gels_ragged_kernel");
         if (rc) return rc;
     }
