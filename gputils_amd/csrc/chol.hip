@@ -2,7 +2,10 @@
 //
 // Replaces cusolverDn?potrfBatched / ?potrsBatched of GPUtils'
 // CholeskyBatchFactoriser (include/tensor.cuh:2098-2197) and adds a fused
-// factor+solve and a mixed-precision (FP32 factor, FP64 refinement) solve.
+// factor+solve.  The mixed-precision solve (FP32 factor, FP64 refinement)
+// is in chol_mixed.hip and n <= 32 in chol_small.hip; its C entry point is
+// here, with this file's FP64 kernel as the fallback for systems that do
+// not converge.
 //
 // Design (one 256-thread workgroup = 4 wavefronts per matrix):
 //  * The lower block triangle of the matrix lives in LDS as 16x16 blocks
@@ -488,204 +491,6 @@ __global__ __launch_bounds__(kThreads) void potrs_global_kernel(int n, int nrhs,
         }
     }
     if (threadIdx.x == 0 && info != nullptr) info[b] = 0;
-}
-
-// --------------------------------------------------- mixed precision --
-constexpr int kTileLd = 17;  // padded leading dimension of a staged 16x16 FP64 tile
-
-// Coalesced fetch of the lower block (BI, BJ) of the FP64 matrix: lane
-// (i = lane%16, c = lane/16) reads rows 16BI+i of columns 16BJ+c+4u.
-__device__ __forceinline__ void fetch_block(const double *__restrict__ a, int64_t lda, int n, int BI,
-                                            int BJ, double (&v)[4]) {
-    const int lane = threadIdx.x & 63;
-    const int row = 16 * BI + (lane & 15);
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        const int col = 16 * BJ + (lane >> 4) + 4 * u;
-        v[u] = (row < n && col < n && row >= col) ? a[row + (int64_t)col * lda] : 0.0;
-    }
-}
-
-// FP64 residual R = B - A X for one 4-column chunk.  The wave owning block
-// row I stages the eight blocks of that row (lower blocks (I,J) for J <= I,
-// (J,I) for J > I) one at a time through its private LDS tile with
-// coalesced loads, prefetching the next block into registers.  Lane
-// (a = lane%16, g = lane/16) accumulates row 16I+a over columns
-// 16J+4g..16J+4g+3; two xor shuffles sum the four column quarters.
-// XD is [row][4] FP64 (LDS); b holds this lane's b[16I+a][g] for its block
-// rows; the FP32 residual goes to R (column-major, ld npad).
-__device__ __forceinline__ void residual_fp64(const double *__restrict__ a, int64_t lda, int n, int nb,
-                                              const double *XD, const double (&bv)[2], float *R,
-                                              double *tile, double (*s_red)[3][kRhsChunk],
-                                              double *s_amax) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int ai = lane & 15, g = lane >> 4;
-    const int npad = 16 * nb;
-    double rmax = 0.0, xmax = 0.0, bmax = 0.0, amax = 0.0;
-    int slot = 0;
-    for (int I = wave; I < nb; I += kWaves, ++slot) {
-        const int p = 16 * I + ai;
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, arow = 0.0;
-        double v[4];
-        fetch_block(a, lda, n, I >= 0 ? I : 0, 0, v);  // block (I,0)
-        for (int J = 0; J < nb; ++J) {
-            // stage block J's lower storage, prefetch block J+1
-#pragma unroll
-            for (int u = 0; u < 4; ++u) tile[ai + kTileLd * ((lane >> 4) + 4 * u)] = v[u];
-            if (J + 1 < nb) {
-                const int Jn = J + 1;
-                fetch_block(a, lda, n, Jn <= I ? I : Jn, Jn <= I ? Jn : I, v);
-            }
-            __builtin_amdgcn_wave_barrier();
-            // element (16I+a, 16J+c): stored at (a,c) of block (I,J) if J < I,
-            // at (c,a) of block (J,I) if J > I, and at (max,min) if J == I
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int c = 4 * g + u;
-                const int r_ = (J < I || (J == I && c <= ai)) ? ai : c;
-                const int c_ = (J < I || (J == I && c <= ai)) ? c : ai;
-                const double av = tile[r_ + kTileLd * c_];
-                const double *x = XD + 4 * (16 * J + c);
-                s0 += av * x[0];
-                s1 += av * x[1];
-                s2 += av * x[2];
-                s3 += av * x[3];
-                arow += fabs(av);
-            }
-            __builtin_amdgcn_wave_barrier();
-        }
-#pragma unroll
-        for (int off = 16; off <= 32; off <<= 1) {
-            s0 += __shfl_xor(s0, off, kWave);
-            s1 += __shfl_xor(s1, off, kWave);
-            s2 += __shfl_xor(s2, off, kWave);
-            s3 += __shfl_xor(s3, off, kWave);
-            arow += __shfl_xor(arow, off, kWave);
-        }
-        const double sg = (g == 0) ? s0 : (g == 1) ? s1 : (g == 2) ? s2 : s3;
-        const double b = bv[slot];
-        const double r = b - sg;
-        R[p + npad * g] = (float)r;
-        rmax = fmax(rmax, fabs(r));
-        bmax = fmax(bmax, fabs(b));
-        xmax = fmax(xmax, fabs(XD[4 * p + g]));
-        amax = fmax(amax, arow);
-    }
-#pragma unroll
-    for (int off = 1; off < 16; off <<= 1) {
-        rmax = fmax(rmax, __shfl_xor(rmax, off, kWave));
-        xmax = fmax(xmax, __shfl_xor(xmax, off, kWave));
-        bmax = fmax(bmax, __shfl_xor(bmax, off, kWave));
-        amax = fmax(amax, __shfl_xor(amax, off, kWave));
-    }
-    if (ai == 0) {
-        s_red[wave][0][g] = rmax;
-        s_red[wave][1][g] = xmax;
-        s_red[wave][2][g] = bmax;
-    }
-    if (lane == 0) s_amax[wave] = amax;
-}
-
-// FP32 factor in LDS, FP64 iterate in LDS, right-hand sides in registers,
-// FP64 residual from the FP64 matrix in HBM/L2.  Stopping test per column:
-// ||b - A x||_inf <= tol (||A||_inf ||x||_inf + ||b||_inf).
-template <int NB>
-__global__ __launch_bounds__(kThreads) void posv_mixed_kernel(
-    int n, int nrhs, const double *__restrict__ A, int64_t lda, int64_t strideA,
-    const double *__restrict__ B, int64_t ldb, int64_t strideB, double *__restrict__ X,
-    int64_t ldx, int64_t strideX, double tol, int max_iter, int *iters, int *info, int64_t batch) {
-    static_assert(NB <= 2 * kWaves, "two block rows per wavefront at most");
-    __shared__ float L[tri(NB) * 256];
-    __shared__ float R[16 * NB * kRhsChunk];                                // FP32 rhs / correction, col-major
-    __shared__ __attribute__((aligned(16))) double XD[16 * NB * kRhsChunk];  // FP64 iterate [row][4]
-    __shared__ double TILE[kWaves][16 * kTileLd];                           // per-wave staging tile
-    __shared__ double s_red[kWaves][3][kRhsChunk];
-    __shared__ double s_amax[kWaves];
-    __shared__ int s_flag;
-    const int64_t b = blockIdx.x;
-    if (b >= batch) return;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int nb = (n + 15) >> 4;
-    const int npad = 16 * nb;
-    const double *a = A + b * strideA;
-    const double *bb = B + b * strideB;
-    double *xx = X + b * strideX;
-    if (threadIdx.x == 0) info[b] = 0;
-
-    load_lower(a, lda, n, nb, L);
-    __syncthreads();
-    const int fail = factor_blocked(L, nb, n, (float *)nullptr, 0, &s_flag);
-    if (fail) {
-        if (threadIdx.x == 0) iters[b] = -1;
-        return;
-    }
-    int used = 0;
-    for (int r0 = 0; r0 < nrhs; r0 += kRhsChunk) {
-        const int nr = (nrhs - r0 < kRhsChunk) ? (nrhs - r0) : kRhsChunk;
-        // b of this lane's rows (block rows wave, wave+4) and column g
-        double bv[2];
-        {
-            const int g = lane >> 4;
-#pragma unroll
-            for (int s2 = 0; s2 < 2; ++s2) {
-                const int p = 16 * (wave + kWaves * s2) + (lane & 15);
-                bv[s2] = (p < n && g < nr) ? bb[p + (int64_t)(r0 + g) * ldb] : 0.0;
-            }
-        }
-        __syncthreads();
-        {
-            const int g = lane >> 4;
-#pragma unroll
-            for (int s2 = 0; s2 < 2; ++s2) {
-                const int p = 16 * (wave + kWaves * s2) + (lane & 15);
-                if (p < npad) {
-                    R[p + npad * g] = (float)bv[s2];
-                    XD[4 * p + g] = 0.0;
-                }
-            }
-        }
-        int it = 0;
-        bool converged = false;
-        for (;;) {
-            // correction: solve L L^T d = R (FP32), x += d (FP64)
-            __syncthreads();
-            solve_blocked(L, nb, R);
-            for (int e = threadIdx.x; e < npad * kRhsChunk; e += kThreads)
-                XD[e] += (double)R[(e >> 2) + npad * (e & 3)];
-            __syncthreads();
-            residual_fp64(a, lda, n, nb, XD, bv, R, TILE[wave], s_red, s_amax);
-            __syncthreads();
-            // every thread evaluates the same test (uniform branch)
-            double anorm = 0.0;
-            for (int w = 0; w < kWaves; ++w) anorm = fmax(anorm, s_amax[w]);
-            bool ok = true;
-            for (int c = 0; c < nr; ++c) {
-                double rn = 0.0, xn = 0.0, bn = 0.0;
-                for (int w = 0; w < kWaves; ++w) {
-                    rn = fmax(rn, s_red[w][0][c]);
-                    xn = fmax(xn, s_red[w][1][c]);
-                    bn = fmax(bn, s_red[w][2][c]);
-                }
-                ok = ok && (rn <= tol * (anorm * xn + bn));
-            }
-            if (ok) {
-                converged = true;
-                break;
-            }
-            if (it >= max_iter) break;
-            ++it;
-        }
-        used = (it > used) ? it : used;
-        if (!converged) {
-            used = -1;
-            break;
-        }
-        for (int e = threadIdx.x; e < npad * kRhsChunk; e += kThreads) {
-            const int row = e >> 2, c = e & 3;
-            if (row < n && c < nr) xx[row + (int64_t)(r0 + c) * ldx] = XD[e];
-        }
-    }
-    if (threadIdx.x == 0) iters[b] = used;
 }
 
 // ----------------------------------------------------------- dispatch --
