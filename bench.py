@@ -457,8 +457,12 @@ def bench_svd_nullspace(a):
     tb = statistics.median(_events_time_ms(build, 5))
     N, P, rank = build()
     At = api.transpose_batched(dA)
-    S, U, VT, info, sw = api.gesvd_batched(At, compute_u=True, sweeps=True)
-    tsvd = statistics.median(_events_time_ms(lambda: api.gesvd_batched(At, compute_u=True), 5))
+    # the SVD as the nullspace build runs it (no right singular vectors:
+    # sweeps on R^T), and the full SVD with VT for reference
+    S, U, VT, info, sw = api.gesvd_batched(At, compute_u=True, sweeps=True, compute_v=False)
+    tsvd = statistics.median(_events_time_ms(lambda: api.gesvd_batched(At, compute_u=True, compute_v=False), 5))
+    _, _, _, _, sw_full = api.gesvd_batched(At, compute_u=True, sweeps=True)
+    tsvd_full = statistics.median(_events_time_ms(lambda: api.gesvd_batched(At, compute_u=True), 5))
     torch.cuda.synchronize()
     b = torch.from_numpy(x).cuda()
     reps = 1000
@@ -512,6 +516,7 @@ def bench_svd_nullspace(a):
                            "P read (96*96*4 B) + b read and written per matrix; P (75.5 MB) fits in the 256 MB "
                            "Infinity Cache, so graph replays can exceed the HBM rate"),
                  cpu, build={"nullspace_build_ms": tb, "gesvd_ms": tsvd, "max_sweeps": int(sw.max()),
+                             "gesvd_with_vt_ms": tsvd_full, "max_sweeps_with_vt": int(sw_full.max()),
                              "ranks": sorted(set(rank.cpu().tolist())), "info_nonzero": int((info != 0).sum())},
                  accuracy={"max_rel_A_Px": rel}, project_eager_event_ms=ev)
 

@@ -288,15 +288,17 @@ class RaggedLeastSquares:
 
 
 # ------------------------------------------------------------------ SVD --
-def gesvd_batched(A: torch.Tensor, compute_u: bool = False, sweeps: bool = False):
+def gesvd_batched(A: torch.Tensor, compute_u: bool = False, sweeps: bool = False, compute_v: bool = True):
     """SVD of every tall m x n matrix of A (tensor layout (k, n, m), m >= n,
-    n <= 64).  Returns (S (k, 1, n), U (k, m, m) or None, VT (k, n, n), info
-    (k,)[, sweeps (k,)]).  A is not modified."""
+    n <= 64).  Returns (S (k, 1, n), U (k, m, m) or None, VT (k, n, n) or
+    None, info (k,)[, sweeps (k,)]).  A is not modified.  compute_v=False
+    skips the right singular vectors (jobvt 'N'): the sweeps then run on
+    R^T, giving the same S and column spaces of U (signs may differ)."""
     _check_dev(A, "A")
     k, n, m = A.shape
     dev, dt = A.device, A.dtype
     S = torch.empty((k, 1, n), dtype=dt, device=dev)
-    VT = torch.empty((k, n, n), dtype=dt, device=dev)
+    VT = torch.empty((k, n, n), dtype=dt, device=dev) if compute_v else None
     U = torch.empty((k, m, m), dtype=dt, device=dev) if compute_u else None
     info = torch.empty(k, dtype=torch.int32, device=dev)
     sw = torch.empty(k, dtype=torch.int32, device=dev)
@@ -306,7 +308,8 @@ def gesvd_batched(A: torch.Tensor, compute_u: bool = False, sweeps: bool = False
         raise GputilsError(f"unsupported SVD shape m={m}, n={n}")
     work = torch.empty(max(wbytes, 1), dtype=torch.uint8, device=dev) if wbytes > 0 else None
     check(getattr(L, f"gputils_{_pfx(A)}gesvd_batched")(
-        m, n, _ptr(A), m, m * n, _ptr(S), n, _ptr(U) if U is not None else None, m, m * m, _ptr(VT), n, n * n,
+        m, n, _ptr(A), m, m * n, _ptr(S), n, _ptr(U) if U is not None else None, m, m * m,
+        _ptr(VT) if VT is not None else None, n, n * n,
         _ptr(work) if work is not None else None, _ptr(info), _ptr(sw), k, stream_handle()), "gesvd_batched")
     return (S, U, VT, info, sw) if sweeps else (S, U, VT, info)
 
@@ -333,7 +336,7 @@ def nullspace_batched(A: torch.Tensor, eps: float = 1e-6):
     if m > n:
         raise GputilsError("nullspace expects square or fat matrices (m <= n)")
     At = transpose_batched(A)            # (k, m, n): n x m matrices
-    S, U, _, _ = gesvd_batched(At, compute_u=True)
+    S, U, _, _ = gesvd_batched(At, compute_u=True, compute_v=False)
     rank = svd_rank_batched(S, eps)
     N = torch.empty((k, n, n), dtype=A.dtype, device=A.device)
     P = torch.empty((k, n, n), dtype=A.dtype, device=A.device)

@@ -145,9 +145,16 @@ __global__ __launch_bounds__(kThreads) void gesvd_kernel(int m, int n, const T *
     }
     __syncthreads();
     hh::hqr_factor<double, double>(G, m, m, n, tau, s_tau);
+    // VT == NULL (no right singular vectors): the sweeps run on W = R^T, so
+    // that the accumulated rotations X are the left singular vectors of R
+    // (R^T Z = V_R Sigma  =>  R = Z Sigma V_R^T): no completion is needed for
+    // zero sigma, and on this matrix type (the nullspace builds' A^T, rank
+    // deficient) they converge in fewer sweeps.
+    const bool on_rt = (VT == nullptr);
     for (int e = tid; e < n * n; e += kThreads) {
         const int i = e % n, j = e / n;
-        W[e] = (i <= j) ? G[i + (int64_t)j * m] : 0.0;
+        if (on_rt) W[e] = (j <= i) ? G[j + (int64_t)i * m] : 0.0;
+        else W[e] = (i <= j) ? G[i + (int64_t)j * m] : 0.0;
         X[e] = (i == j) ? 1.0 : 0.0;
     }
     __syncthreads();
@@ -255,10 +262,12 @@ __global__ __launch_bounds__(kThreads) void gesvd_kernel(int m, int n, const T *
     __syncthreads();
     T *s_out = S + b * strideS;
     for (int c = tid; c < n; c += kThreads) s_out[c] = (T)sig[ord[c]];
-    T *vt = VT + b * strideVT;
-    for (int e = tid; e < n * n; e += kThreads) {
-        const int rr = e % n, i = e / n, j = ord[rr];
-        vt[rr + (int64_t)i * ldvt] = (T)(sgnv[j] * X[i + j * n]);
+    if (!on_rt) {
+        T *vt = VT + b * strideVT;
+        for (int e = tid; e < n * n; e += kThreads) {
+            const int rr = e % n, i = e / n, j = ord[rr];
+            vt[rr + (int64_t)i * ldvt] = (T)(sgnv[j] * X[i + j * n]);
+        }
     }
     if (tid == 0) {
         info[b] = converged ? 0 : 1;
@@ -268,10 +277,20 @@ __global__ __launch_bounds__(kThreads) void gesvd_kernel(int m, int n, const T *
     __syncthreads();
 
     // U_R in output order (X is free once VT is written)
-    const int nz = *s_nz;
-    for (int e = tid; e < n * n; e += kThreads) {
-        const int i = e % n, c = e / n, j = ord[c];
-        X[e] = (c < nz) ? sgnv[j] * W[i + j * n] / sig[j] : 0.0;
+    const int nz = on_rt ? n : *s_nz;
+    if (on_rt) {
+        // U_R = the rotations, columns reordered (through W, which is free)
+        for (int e = tid; e < n * n; e += kThreads) {
+            const int i = e % n, c = e / n, j = ord[c];
+            W[e] = sgnv[j] * X[i + j * n];
+        }
+        __syncthreads();
+        for (int e = tid; e < n * n; e += kThreads) X[e] = W[e];
+    } else {
+        for (int e = tid; e < n * n; e += kThreads) {
+            const int i = e % n, c = e / n, j = ord[c];
+            X[e] = (c < nz) ? sgnv[j] * W[i + j * n] / sig[j] : 0.0;
+        }
     }
     __syncthreads();
     if (wave == 0 && nz < n) {
@@ -448,14 +467,13 @@ int gesvd_batched(int m, int n, const T *A, int64_t lda, int64_t strideA, T *S, 
     GPUTILS_ARG(strideS >= 0, 7, "negative strideS");
     GPUTILS_ARG(U == nullptr || ldu >= m, 9, "ldu < m");
     GPUTILS_ARG(strideU >= 0, 10, "negative strideU");
-    GPUTILS_ARG(ldvt >= n, 12, "ldvt < n");
+    GPUTILS_ARG(VT == nullptr || ldvt >= n, 12, "ldvt < n");
     GPUTILS_ARG(strideVT >= 0, 13, "negative strideVT");
     GPUTILS_ARG(info != nullptr, 15, "info is NULL");
     GPUTILS_ARG(batch >= 0 && batch <= 0x7fffffff, 17, "batch out of range");
     if (batch == 0) return 0;
     GPUTILS_ARG(A != nullptr, 3, "A is NULL");
     GPUTILS_ARG(S != nullptr, 6, "S is NULL");
-    GPUTILS_ARG(VT != nullptr, 11, "VT is NULL");
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (resident_fits(m, n))
         return launch_gesvd<T, true>(m, n, A, lda, strideA, S, strideS, U, ldu, strideU, VT, ldvt, strideVT,

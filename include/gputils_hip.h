@@ -228,7 +228,11 @@ int gputils_graph_destroy(gputils_graph_t graph);
  * A is read only.  FP64 arithmetic inside for both types.  info[i] = 0, or
  * 1 if the Jacobi sweeps did not converge; sweeps (may be NULL) receives the
  * sweep count.  work: gputils_gesvd_workspace_bytes(m, n, batch) bytes of
- * device memory (0 bytes, work may be NULL, when the matrix fits in LDS). */
+ * device memory (0 bytes, work may be NULL, when the matrix fits in LDS).
+ * VT may be NULL (jobvt 'N', no reference counterpart; used by the nullspace
+ * construction): the sweeps then run on R^T and U_R is the accumulated
+ * rotation matrix, which gives the same S and the same column spaces of U
+ * (column signs may differ from the VT != NULL result). */
 int64_t gputils_gesvd_workspace_bytes(int m, int n, int64_t batch);
 int gputils_sgesvd_batched(int m, int n, const float *A, int64_t lda, int64_t strideA, float *S, int64_t strideS,
                            float *U, int64_t ldu, int64_t strideU, float *VT, int64_t ldvt, int64_t strideVT,

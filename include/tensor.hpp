@@ -1070,19 +1070,38 @@ public:
         if (m > n) throw std::invalid_argument("[nullspace] I was expecting a square or fat matrix");
         m_nullspace = std::make_unique<DTensor<T>>(n, n, nMats, true);
         m_projOp = std::make_unique<DTensor<T>>(n, n, nMats, true);
+        if (m > 64) throw std::invalid_argument("[nullspace] at most 64 rows are supported");
         DTensor<T> aTranspose = a.tr();
-        Svd<T> svd(aTranspose, true);
-        svd.factorise();
-        const DTensor<unsigned int> &rank = svd.rank();
-        std::shared_ptr<DTensor<T>> U = svd.leftSingularVectors().value();
-        const int64_t nn = (int64_t)n * n;
-        if constexpr (std::is_same_v<T, double>)
-            gpuErrChk(gputils_dnullspace_build_batched((int)n, U->raw(), n, nn, rank.raw(), m_nullspace->raw(), n, nn,
-                                                       m_projOp->raw(), n, nn, (int64_t)nMats, aTranspose.stream()));
-        if constexpr (std::is_same_v<T, float>)
-            gpuErrChk(gputils_snullspace_build_batched((int)n, U->raw(), n, nn, rank.raw(), m_nullspace->raw(), n, nn,
-                                                       m_projOp->raw(), n, nn, (int64_t)nMats, aTranspose.stream()));
-        gpuErrChk(gputils_stream_synchronize(aTranspose.stream()));
+        // SVD of a' (n x m) without right singular vectors: the sweeps run on
+        // R^T and U comes from the accumulated rotations (same subspaces)
+        DTensor<T> S(m, 1, nMats), U(n, n, nMats);
+        DTensor<int> info(1, 1, nMats);
+        DTensor<unsigned int> rank(1, 1, nMats, true);
+        const int64_t nn = (int64_t)n * n, k = (int64_t)nMats;
+        const int64_t wb = gputils_gesvd_workspace_bytes((int)n, (int)m, k);
+        if (wb < 0) throw std::invalid_argument("[nullspace] unsupported matrix size");
+        void *work = nullptr;
+        if (wb > 0) gpuErrChk(Session::getInstance().allocate(&work, (size_t)wb));
+        void *st = aTranspose.stream();
+        if constexpr (std::is_same_v<T, double>) {
+            gpuErrChk(gputils_dgesvd_batched((int)n, (int)m, aTranspose.raw(), n, (int64_t)n * m, S.raw(), m, U.raw(),
+                                             n, nn, nullptr, m, (int64_t)m * m, work, info.raw(), nullptr, k, st));
+            gpuErrChk(gputils_dsvd_rank_batched((int)m, S.raw(), m, 1e-6, rank.raw(), k, st));
+            gpuErrChk(gputils_dnullspace_build_batched((int)n, U.raw(), n, nn, rank.raw(), m_nullspace->raw(), n, nn,
+                                                       m_projOp->raw(), n, nn, k, st));
+        }
+        if constexpr (std::is_same_v<T, float>) {
+            gpuErrChk(gputils_sgesvd_batched((int)n, (int)m, aTranspose.raw(), n, (int64_t)n * m, S.raw(), m, U.raw(),
+                                             n, nn, nullptr, m, (int64_t)m * m, work, info.raw(), nullptr, k, st));
+            gpuErrChk(gputils_ssvd_rank_batched((int)m, S.raw(), m, 1e-6f, rank.raw(), k, st));
+            gpuErrChk(gputils_snullspace_build_batched((int)n, U.raw(), n, nn, rank.raw(), m_nullspace->raw(), n, nn,
+                                                       m_projOp->raw(), n, nn, k, st));
+        }
+        gpuErrChk(gputils_stream_synchronize(st));
+        if (work) {
+            gputils_free(work);
+            Session::getInstance().incrementAllocatedBytes(-wb);
+        }
     }
 
     DTensor<T> const &nullspace() const { return *m_nullspace; }

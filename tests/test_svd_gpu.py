@@ -144,6 +144,41 @@ def test_svd_rank_deficient_and_zero(oracle, dtype):
     assert np.allclose(U[1], np.eye(20)) and np.allclose(VT[1], np.eye(8))
 
 
+@pytest.mark.parametrize("m,n", [(96, 48), (20, 8), (64, 64), (5, 1), (30, 17)])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_svd_without_vt_same_values_and_subspaces(m, n, dtype):
+    """VT = NULL (the nullspace builds' mode) runs the sweeps on R^T: the
+    singular values equal the full SVD's, U is orthogonal, ||A^T u_j|| =
+    sigma_j for the nonzero sigma, and the projectors onto the leading and
+    the trailing columns of U agree with the full SVD's (the column signs
+    may differ).  The second matrix is rank deficient."""
+    r = np.random.default_rng(100 * m + n)
+    rk = max(1, n // 2)
+    A = np.stack([r.standard_normal((m, n)), r.standard_normal((m, rk)) @ r.standard_normal((rk, n))])
+    if dtype == torch.float32:
+        A = A.astype(np.float32).astype(np.float64)
+    tA = tensor_of(A, dtype)
+    S1, U1, VT1, info1 = api.gesvd_batched(tA, compute_u=True)
+    S2, U2, VT2, info2 = api.gesvd_batched(tA, compute_u=True, compute_v=False)
+    assert VT2 is None
+    assert info1.tolist() == [0, 0] and info2.tolist() == [0, 0]
+    tol = 1e-12 if dtype == torch.float64 else 2e-6
+    S1, S2 = S1.cpu().double().numpy()[:, 0, :], S2.cpu().double().numpy()[:, 0, :]
+    U1 = np.swapaxes(U1.cpu().double().numpy(), 1, 2)
+    U2 = np.swapaxes(U2.cpu().double().numpy(), 1, 2)
+    for b in range(2):
+        smax = max(S1[b, 0], 1.0)
+        assert np.max(np.abs(S1[b] - S2[b])) <= 10 * tol * smax
+        assert np.allclose(U2[b].T @ U2[b], np.eye(m), atol=100 * tol)
+        rb = int(np.sum(S1[b] > 1e-6 * smax))
+        for j in range(rb):
+            assert abs(np.linalg.norm(A[b].T @ U2[b][:, j]) - S2[b, j]) <= 100 * tol * smax
+        for lo, hi in ((0, rb), (rb, m)):
+            P1 = U1[b][:, lo:hi] @ U1[b][:, lo:hi].T
+            P2 = U2[b][:, lo:hi] @ U2[b][:, lo:hi].T
+            assert np.allclose(P1, P2, atol=1e3 * tol), (b, lo, hi, np.abs(P1 - P2).max())
+
+
 def test_svd_argument_checks():
     A = torch.zeros((2, 3, 5), dtype=torch.float64, device="cuda")   # 5 x 3: fine
     api.gesvd_batched(A)
