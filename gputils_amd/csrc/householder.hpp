@@ -1,8 +1,8 @@
 // householder.hpp -- wavefront-level Householder building blocks shared by
 // the QR / least-squares and SVD kernels (LAPACK dlarfg / dlarf conventions:
 // beta = -sign(alpha)*||[alpha; x]||, tau = (beta - alpha)/beta,
-// v = [1; x / (alpha - beta)]).  A 256-thread workgroup is assumed; column c
-// of a matrix being factorised is owned by wavefront c % 4.
+// v = [1; x / (alpha - beta)]).  The factorisation spreads the columns over
+// the workgroup's wavefronts (four unless a kernel asks for another count).
 #pragma once
 
 #include "common.hpp"
@@ -117,22 +117,22 @@ __device__ __forceinline__ void apply_reflector_col(T *A, int64_t lda, int m, in
     if (lane == 0) x[j] -= tw;
 }
 
-// x_c <- (I - tau v v^T) x_c for the columns c = c0, c0 + 4, ... < n (the
-// columns one wavefront owns), v = [1; A[j+1:m, j]], four columns at a
+// x_c <- (I - tau v v^T) x_c for the columns c = c0, c0 + NW, ... < n (the
+// columns one of NW wavefronts owns), v = [1; A[j+1:m, j]], four columns at a
 // time: their dot products are accumulated together and their reductions
 // overlap, so the wavefront waits for one set of reductions per four
 // columns instead of one per column.  Past the last column the slots work
 // on the first owned column with tau w = 0.
-template <typename T, typename DOT>
+template <typename T, typename DOT, int NW = kWaves>
 __device__ __forceinline__ void apply_reflector_cols(T *A, int64_t lda, int m, int j, T tau, int c0, int n) {
     const int lane = threadIdx.x & 63;
     const T *v = A + (int64_t)j * lda;
-    for (int cq = c0; cq < n; cq += 4 * kWaves) {
+    for (int cq = c0; cq < n; cq += 4 * NW) {
         T *x[4];
         bool ok[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int c = cq + kWaves * q;
+            const int c = cq + NW * q;
             ok[q] = c < n;
             x[q] = A + (int64_t)(ok[q] ? c : cq) * lda;
         }
@@ -168,8 +168,11 @@ __device__ __forceinline__ void apply_reflector_cols(T *A, int64_t lda, int m, i
 // products in DOT.  s_tau: two LDS slots.  Ends with a barrier.  Columns n ..
 // ntot-1 (stored after the factorised ones) receive every reflector
 // without being factorised: a right-hand side stored there ends as Q^T b.
-template <typename T, typename ACC, typename DOT = ACC>
+// NW (a power of two) wavefronts: column c is owned by wavefront c % NW.
+// Each column's arithmetic is the same for every NW.
+template <typename T, typename ACC, typename DOT = ACC, int NW = kWaves>
 __device__ __forceinline__ void hqr_factor(T *A, int64_t lda, int m, int n, T *tau, T *s_tau, int ntot = 0) {
+    static_assert(NW >= 1 && (NW & (NW - 1)) == 0, "NW must be a power of two");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (n <= 0) return;
     if (ntot < n) ntot = n;
@@ -183,7 +186,7 @@ __device__ __forceinline__ void hqr_factor(T *A, int64_t lda, int m, int n, T *t
     __syncthreads();
     for (int j = 0; j < n; ++j) {
         const T t = s_tau[j & 1];
-        if (j + 1 < ntot && wave == ((j + 1) & (kWaves - 1))) {
+        if (j + 1 < ntot && wave == ((j + 1) & (NW - 1))) {
             if (j + 1 < n && m <= 8 * kWave) {
                 const T t1 = update_and_reflect<T, ACC, DOT, 8>(A, lda, m, j, t);
                 if (lane == 0) {
@@ -201,7 +204,8 @@ __device__ __forceinline__ void hqr_factor(T *A, int64_t lda, int m, int n, T *t
                 }
             }
         }
-        if (t != T(0)) apply_reflector_cols<T, DOT>(A, lda, m, j, t, j + 2 + ((wave - (j + 2)) & (kWaves - 1)), ntot);
+        if (t != T(0))
+            apply_reflector_cols<T, DOT, NW>(A, lda, m, j, t, j + 2 + ((wave - (j + 2)) & (NW - 1)), ntot);
         __syncthreads();
     }
 }

@@ -174,11 +174,12 @@ __device__ __forceinline__ void trsv_upper_trans(const T *A, int64_t lda, int n,
 }
 
 // FP64 r_i = base_i - sum_k A[i][k] x[k] for the rows i of this thread
-// (row-major over threads, coalesced over rows); the loads of eight
+// (row-major over the NT threads, coalesced over rows); the loads of eight
 // columns are issued together so that they are in flight at once.
+template <int NT>
 __device__ __forceinline__ void rows_minus_ax(const double *__restrict__ a, int m, int n, const double *x,
                                               const double *base0, const double *base1, double *out) {
-    for (int i = threadIdx.x; i < m; i += kThreads) {
+    for (int i = threadIdx.x; i < m; i += NT) {
         double r = base0[i] - (base1 ? base1[i] : 0.0);
         int k = 0;
         for (; k + 8 <= n; k += 8) {
@@ -205,9 +206,11 @@ __device__ __forceinline__ void rows_minus_ax(const double *__restrict__ a, int 
 // accuracy also for inconsistent systems, where refining x alone stalls at
 // the FP32 level.  x overwrites b[0:n].  Global loads are batched (eight
 // per thread in flight) because one wavefront per SIMD leaves little else
-// to hide their latency.
-template <int MAXE>
-__global__ __launch_bounds__(kThreads) void gels_ragged_kernel(const RaggedItem *__restrict__ items,
+// to hide their latency.  NW wavefronts per workgroup (2, 4, 8 or 16; the plan
+// picks it per bucket); every system's arithmetic is the same for every NW
+// and every item-to-workgroup assignment, so the results are too.
+template <int MAXE, int NW>
+__global__ __launch_bounds__(NW * kWave) void gels_ragged_kernel(const RaggedItem *__restrict__ items,
                                                                const int *__restrict__ wstart,
                                                                const double *__restrict__ A, double *B,
                                                                int steps) {
@@ -220,6 +223,7 @@ __global__ __launch_bounds__(kThreads) void gels_ragged_kernel(const RaggedItem 
     __shared__ double xd[64];
     __shared__ double rd[512];
     __shared__ double bd[512];
+    constexpr int NT = NW * kWave;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int it_end = wstart[blockIdx.x + 1];
     for (int it = wstart[blockIdx.x]; it < it_end; ++it) {
@@ -231,33 +235,33 @@ __global__ __launch_bounds__(kThreads) void gels_ragged_kernel(const RaggedItem 
         {
             const int mn = m * n;
             int e0 = threadIdx.x;
-            for (; e0 + 7 * kThreads < mn; e0 += 8 * kThreads) {
+            for (; e0 + 7 * NT < mn; e0 += 8 * NT) {
                 double v[8];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) v[u] = a[e0 + u * kThreads];
+                for (int u = 0; u < 8; ++u) v[u] = a[e0 + u * NT];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) S[e0 + u * kThreads] = (float)v[u];
+                for (int u = 0; u < 8; ++u) S[e0 + u * NT] = (float)v[u];
             }
-            for (; e0 < mn; e0 += kThreads) S[e0] = (float)a[e0];
+            for (; e0 < mn; e0 += NT) S[e0] = (float)a[e0];
         }
         float *qb = S + m * n;  // b as column n: the factorisation turns it into Q^T b
-        for (int i = threadIdx.x; i < m; i += kThreads) {
+        for (int i = threadIdx.x; i < m; i += NT) {
             const double v = bb[i];
             bd[i] = v;
             qb[i] = (float)v;
         }
         __syncthreads();
-        hqr_factor<float, double, float>(S, m, m, n, tau, s_tau, n + 1);  // FP64 norms, FP32 dots
+        hqr_factor<float, double, float, NW>(S, m, m, n, tau, s_tau, n + 1);  // FP64 norms, FP32 dots
         if (wave == 0) trsv_upper<float>(S, m, n, qb);  // x0 = R^{-1} (Q^T b)[0:n]
         __syncthreads();
-        for (int i = threadIdx.x; i < n; i += kThreads) xd[i] = (double)qb[i];
+        for (int i = threadIdx.x; i < n; i += NT) xd[i] = (double)qb[i];
         __syncthreads();
         // r0 = b - A x0 (FP64, A from global memory)
-        rows_minus_ax(a, m, n, xd, bd, nullptr, rd);
+        rows_minus_ax<NT>(a, m, n, xd, bd, nullptr, rd);
         for (int s = 0; s < steps; ++s) {
             __syncthreads();
             // f = b - r - A x  -> c (FP32), staged through h-sized chunks of rows
-            for (int i = threadIdx.x; i < m; i += kThreads) {
+            for (int i = threadIdx.x; i < m; i += NT) {
                 double f = bd[i] - rd[i];
                 int k = 0;
                 for (; k + 8 <= n; k += 8) {
@@ -272,7 +276,7 @@ __global__ __launch_bounds__(kThreads) void gels_ragged_kernel(const RaggedItem 
             }
             // g = -A^T r -> h (FP32); one wavefront per column, eight rows
             // per lane in flight
-            for (int j = wave; j < n; j += kWaves) {
+            for (int j = wave; j < n; j += NW) {
                 const double *col = a + (int64_t)j * m;
                 double g = 0.0;
                 int i0 = lane;
@@ -293,7 +297,7 @@ __global__ __launch_bounds__(kThreads) void gels_ragged_kernel(const RaggedItem 
             if (wave == 0) trsv_upper_trans<float>(S, m, n, h);                 // h = R^{-T} g
             if (wave == 1) apply_q_vec<float, float>(S, m, m, n, tau, c, true);  // c = Q^T f
             __syncthreads();
-            for (int i = threadIdx.x; i < n; i += kThreads) {
+            for (int i = threadIdx.x; i < n; i += NT) {
                 dx[i] = c[i] - h[i];
                 c[i] = h[i];
             }
@@ -304,11 +308,38 @@ __global__ __launch_bounds__(kThreads) void gels_ragged_kernel(const RaggedItem 
             }
             if (wave == 1) apply_q_vec<float, float>(S, m, m, n, tau, c, false);  // c = dr = Q [h; d2]
             __syncthreads();
-            for (int i = threadIdx.x; i < m; i += kThreads) rd[i] += (double)c[i];
+            for (int i = threadIdx.x; i < m; i += NT) rd[i] += (double)c[i];
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < n; i += kThreads) bb[i] = xd[i];
+        for (int i = threadIdx.x; i < n; i += NT) bb[i] = xd[i];
     }
+}
+
+// Launch one bucket of a ragged plan: the LDS capacity MAXE and the
+// wavefronts per workgroup pick the instantiation.
+template <int MAXE>
+void launch_ragged_waves(int waves, int grid, hipStream_t st, const RaggedItem *items, const int *wstart,
+                         const double *A, double *B, int steps) {
+    if (waves == 2)
+        hipLaunchKernelGGL((gels_ragged_kernel<MAXE, 2>), dim3(grid), dim3(2 * kWave), 0, st, items, wstart, A, B, steps);
+    else if (waves == 16)
+        hipLaunchKernelGGL((gels_ragged_kernel<MAXE, 16>), dim3(grid), dim3(16 * kWave), 0, st, items, wstart, A, B, steps);
+    else if (waves == 8)
+        hipLaunchKernelGGL((gels_ragged_kernel<MAXE, 8>), dim3(grid), dim3(8 * kWave), 0, st, items, wstart, A, B, steps);
+    else
+        hipLaunchKernelGGL((gels_ragged_kernel<MAXE, 4>), dim3(grid), dim3(4 * kWave), 0, st, items, wstart, A, B, steps);
+}
+
+int launch_ragged(int maxe, int waves, int grid, hipStream_t st, const RaggedItem *items, const int *wstart,
+                  const double *A, double *B, int steps) {
+    switch (maxe) {
+    case 2048: launch_ragged_waves<2048>(waves, grid, st, items, wstart, A, B, steps); break;
+    case 4096: launch_ragged_waves<4096>(waves, grid, st, items, wstart, A, B, steps); break;
+    case 8192: launch_ragged_waves<8192>(waves, grid, st, items, wstart, A, B, steps); break;
+    case 16384: launch_ragged_waves<16384>(waves, grid, st, items, wstart, A, B, steps); break;
+    default: launch_ragged_waves<32768>(waves, grid, st, items, wstart, A, B, steps); break;
+    }
+    return launch_check("gels_ragged_kernel");
 }
 
 template <typename T>
@@ -355,18 +386,88 @@ int geqrf_dispatch(int64_t m, int64_t n, T *A, int64_t lda, int64_t strideA, T *
 using namespace gputils;
 
 // ---------------------------------------------------------- ragged plan --
+// The systems are grouped into buckets by LDS need (FP32 m*n elements);
+// each bucket is one launch whose launch shape (wavefronts per workgroup,
+// workgroups per CU) the plan holds and the tuner can change.  Workgroup w
+// of a bucket solves a contiguous range of the bucket's items, filled by a
+// longest-processing-time assignment of the cost-sorted systems.
 struct gputils_gels_plan {
-    int64_t count = 0;
-    qr::RaggedItem *d_items = nullptr;
-    int *d_wstart = nullptr;  // per bucket: grid + 1 item offsets, one range per workgroup
     struct Bucket {
         int maxe;
-        int begin, count;
-        int grid;
-        int wbase;  // first entry of this bucket's offsets in d_wstart
+        int begin, count;  // items begin .. begin + count - 1 of d_items
+        int wbase;         // first of this bucket's count + 1 slots in d_wstart
+        int waves;         // wavefronts per workgroup: 2, 4, 8 or 16
+        int per_cu;        // workgroups per CU the grid is sized for
+        int grid;          // min(count, CUs * per_cu)
+        std::vector<int> order;  // the bucket's systems, most expensive first
     };
+    int64_t count = 0;
+    int cus = 256;
+    qr::RaggedItem *d_items = nullptr;
+    int *d_wstart = nullptr;  // per bucket: grid + 1 absolute item offsets
+    std::vector<int32_t> m, n;
+    std::vector<int64_t> offA, offB;
     std::vector<Bucket> buckets;
+    int64_t extentB = 0;  // max over systems of offB + m
 };
+
+namespace {
+
+constexpr int kRaggedBucketMax[] = {2048, 4096, 8192, 16384, 32768};
+constexpr int kRaggedBuckets = 5;
+
+// LDS of gels_ragged_kernel<MAXE, *>: the FP32 matrix and b column plus
+// the fixed per-system vectors (s_tau 8, tau 256, c 2048, h 256, dx 256,
+// xd 512, rd 4096, bd 4096 bytes)
+int ragged_lds_bytes(int maxe) { return (maxe + 512) * 4 + 11528; }
+
+// most workgroups of one bucket that a CU can hold at once: LDS (160 KiB)
+// and 32 wavefronts per CU
+int ragged_max_per_cu(int maxe, int waves) {
+    return std::max(1, std::min(160 * 1024 / ragged_lds_bytes(maxe), 32 / waves));
+}
+
+// estimated cost of one system: the factorisation's m n^2 work spread over
+// a wavefront plus the n-step serial chains of the solves
+double ragged_cost(int m, int n) { return (double)m * n * n / 64.0 + 2000.0 * n; }
+
+// Assign the bucket's items to its grid (longest processing time first:
+// each system, most expensive first, goes to the least-loaded workgroup)
+// and upload the items and range boundaries.
+int ragged_upload_bucket(gputils_gels_plan *p, const gputils_gels_plan::Bucket &bk) {
+    std::vector<std::vector<int>> bins((size_t)bk.grid);
+    std::priority_queue<std::pair<double, int>, std::vector<std::pair<double, int>>,
+                        std::greater<std::pair<double, int>>> heap;
+    for (int w = 0; w < bk.grid; ++w) heap.push({0.0, w});
+    for (int o : bk.order) {
+        auto top = heap.top();
+        heap.pop();
+        bins[(size_t)top.second].push_back(o);
+        heap.push({top.first + ragged_cost(p->m[o], p->n[o]), top.second});
+    }
+    std::vector<qr::RaggedItem> items;
+    std::vector<int> wstart;
+    items.reserve((size_t)bk.count);
+    wstart.reserve((size_t)bk.grid + 1);
+    for (int w = 0; w < bk.grid; ++w) {
+        wstart.push_back(bk.begin + (int)items.size());
+        for (int o : bins[(size_t)w]) items.push_back({p->m[o], p->n[o], p->offA[o], p->offB[o]});
+    }
+    wstart.push_back(bk.begin + (int)items.size());
+    GPUTILS_HIP(hipMemcpy(p->d_items + bk.begin, items.data(), sizeof(qr::RaggedItem) * items.size(),
+                          hipMemcpyHostToDevice));
+    GPUTILS_HIP(hipMemcpy(p->d_wstart + bk.wbase, wstart.data(), sizeof(int) * wstart.size(),
+                          hipMemcpyHostToDevice));
+    return GPUTILS_OK;
+}
+
+void ragged_set_shape(const gputils_gels_plan *p, gputils_gels_plan::Bucket &bk, int waves, int per_cu) {
+    bk.waves = waves;
+    bk.per_cu = per_cu;
+    bk.grid = (int)std::min<int64_t>(bk.count, (int64_t)p->cus * per_cu);
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -460,92 +561,59 @@ int gputils_gels_ragged_plan_create(gputils_gels_plan_t *plan, int64_t count, co
     GPUTILS_ARG(count >= 0 && count <= 0x7fffffff, 2, "count out of range");
     GPUTILS_ARG(count == 0 || (m && n && offA && offB), 3, "size/offset arrays are NULL");
     *plan = nullptr;
-    // buckets by LDS need (FP32 m*n elements); items sorted by cost m*n^2, largest first
-    static const int kBucketMax[] = {2048, 4096, 8192, 16384, 32768};
-    constexpr int kNb = 5;
-    std::vector<qr::RaggedItem> items((size_t)count);
-    std::vector<int> order((size_t)count);
     for (int64_t i = 0; i < count; ++i) {
         if (m[i] < 1 || n[i] < 1 || n[i] > m[i] || n[i] > 64 || m[i] > 512) {
             set_error("invalid argument 3: system %lld has m=%d n=%d (need 1 <= n <= m, n <= 64, m <= 512)",
                       (long long)i, m[i], n[i]);
             return -3;
         }
-        order[i] = (int)i;
     }
-    auto bucket_of = [&](int i) {
-        const int64_t e = (int64_t)m[i] * n[i];
-        int bk = 0;
-        while (bk + 1 < kNb && e > kBucketMax[bk]) ++bk;
-        return bk;
-    };
-    // estimated cost of one system: the factorisation's m n^2 work spread
-    // over a wavefront plus the n-step serial chains of the solves
-    auto cost_of = [&](int i) { return (double)m[i] * n[i] * n[i] / 64.0 + 2000.0 * n[i]; };
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-        const int bx = bucket_of(x), by = bucket_of(y);
-        if (bx != by) return bx > by;
-        return cost_of(x) > cost_of(y);
-    });
     auto *p = new gputils_gels_plan();
     p->count = count;
-    int dev = 0, cus = 256;
+    p->m.assign(m, m + count);
+    p->n.assign(n, n + count);
+    p->offA.assign(offA, offA + count);
+    p->offB.assign(offB, offB + count);
+    for (int64_t i = 0; i < count; ++i) p->extentB = std::max(p->extentB, offB[i] + m[i]);
+    int dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
-        cus = prop.multiProcessorCount;
-    // per bucket: workgroups resident at once (the kernel's LDS is the
-    // bucket's FP32 matrix and b column plus 13576 bytes of per-system
-    // vectors), then a longest-processing-time assignment of the
-    // cost-sorted items to them; each workgroup's items are stored
-    // contiguously and wstart holds the range boundaries
-    std::vector<int> sorted_items;
-    std::vector<int> wstart;
-    sorted_items.reserve((size_t)count);
-    int start = 0;
-    for (int bk = kNb - 1; bk >= 0; --bk) {
-        int cnt = 0;
-        while (start + cnt < count && bucket_of(order[start + cnt]) == bk) ++cnt;
-        if (cnt > 0) {
-            const int lds = (kBucketMax[bk] + 512) * 4 + 13576;
-            const int per_cu = std::max(1, std::min(8, 160 * 1024 / lds));
-            const int grid = std::min(cnt, cus * per_cu);
-            std::vector<std::vector<int>> bins((size_t)grid);
-            std::priority_queue<std::pair<double, int>, std::vector<std::pair<double, int>>,
-                                std::greater<std::pair<double, int>>> heap;
-            for (int w = 0; w < grid; ++w) heap.push({0.0, w});
-            for (int k = 0; k < cnt; ++k) {
-                const int o = order[start + k];
-                auto top = heap.top();
-                heap.pop();
-                bins[(size_t)top.second].push_back(o);
-                heap.push({top.first + cost_of(o), top.second});
-            }
-            const int begin = (int)sorted_items.size();
-            p->buckets.push_back({kBucketMax[bk], begin, cnt, grid, (int)wstart.size()});
-            for (int w = 0; w < grid; ++w) {
-                wstart.push_back((int)sorted_items.size());
-                for (int o : bins[(size_t)w]) sorted_items.push_back(o);
-            }
-            wstart.push_back((int)sorted_items.size());
-        }
-        start += cnt;
+        p->cus = prop.multiProcessorCount;
+    // buckets by LDS need, largest first; within a bucket by cost, largest first
+    std::vector<std::vector<int>> members(kRaggedBuckets);
+    for (int i = 0; i < (int)count; ++i) {
+        const int64_t e = (int64_t)m[i] * n[i];
+        int bk = 0;
+        while (bk + 1 < kRaggedBuckets && e > kRaggedBucketMax[bk]) ++bk;
+        members[(size_t)bk].push_back(i);
     }
-    for (int64_t i = 0; i < count; ++i) {
-        const int o = sorted_items[(size_t)i];
-        items[i] = {m[o], n[o], offA[o], offB[o]};
+    int begin = 0;
+    for (int bk = kRaggedBuckets - 1; bk >= 0; --bk) {
+        auto &mem = members[(size_t)bk];
+        if (mem.empty()) continue;
+        std::stable_sort(mem.begin(), mem.end(),
+                         [&](int x, int y) { return ragged_cost(m[x], n[x]) > ragged_cost(m[y], n[y]); });
+        gputils_gels_plan::Bucket b;
+        b.maxe = kRaggedBucketMax[bk];
+        b.begin = begin;
+        b.count = (int)mem.size();
+        b.wbase = begin + (int)p->buckets.size();
+        b.order = std::move(mem);
+        // untuned shape: eight wavefronts, as many workgroups per CU as fit
+        ragged_set_shape(p, b, 8, ragged_max_per_cu(b.maxe, 8));
+        begin += b.count;
+        p->buckets.push_back(std::move(b));
     }
     if (count > 0) {
         hipError_t e = hipMalloc(&p->d_items, sizeof(qr::RaggedItem) * (size_t)count);
-        if (e == hipSuccess) e = hipMalloc(&p->d_wstart, sizeof(int) * wstart.size());
-        if (e == hipSuccess)
-            e = hipMemcpy(p->d_items, items.data(), sizeof(qr::RaggedItem) * (size_t)count, hipMemcpyHostToDevice);
-        if (e == hipSuccess)
-            e = hipMemcpy(p->d_wstart, wstart.data(), sizeof(int) * wstart.size(), hipMemcpyHostToDevice);
-        if (e != hipSuccess) {
+        if (e == hipSuccess) e = hipMalloc(&p->d_wstart, sizeof(int) * ((size_t)count + p->buckets.size()));
+        int rc = e == hipSuccess ? GPUTILS_OK : hip_fail(e, "plan allocation");
+        for (size_t i = 0; rc == GPUTILS_OK && i < p->buckets.size(); ++i) rc = ragged_upload_bucket(p, p->buckets[i]);
+        if (rc != GPUTILS_OK) {
             if (p->d_items) (void)hipFree(p->d_items);
             if (p->d_wstart) (void)hipFree(p->d_wstart);
             delete p;
-            return hip_fail(e, "plan upload");
+            return rc;
         }
     }
     *plan = p;
@@ -560,6 +628,101 @@ int gputils_gels_ragged_plan_destroy(gputils_gels_plan_t plan) {
     return GPUTILS_OK;
 }
 
+int gputils_gels_ragged_plan_describe(gputils_gels_plan_t plan, int32_t *desc, int capacity, int *nbuckets) {
+    GPUTILS_ARG(plan != nullptr, 1, "plan is NULL");
+    GPUTILS_ARG(capacity >= 0, 3, "capacity < 0");
+    GPUTILS_ARG(capacity == 0 || desc != nullptr, 2, "desc is NULL");
+    if (nbuckets) *nbuckets = (int)plan->buckets.size();
+    for (int i = 0; i < capacity && i < (int)plan->buckets.size(); ++i) {
+        const auto &bk = plan->buckets[(size_t)i];
+        const int32_t v[5] = {bk.maxe, bk.count, bk.waves, bk.per_cu, bk.grid};
+        for (int q = 0; q < 5; ++q) desc[5 * i + q] = v[q];
+    }
+    return GPUTILS_OK;
+}
+
+int gputils_gels_ragged_plan_configure(gputils_gels_plan_t plan, int bucket, int waves, int per_cu) {
+    GPUTILS_ARG(plan != nullptr, 1, "plan is NULL");
+    GPUTILS_ARG(bucket >= 0 && bucket < (int)plan->buckets.size(), 2, "bucket out of range");
+    GPUTILS_ARG(waves == 2 || waves == 4 || waves == 8 || waves == 16, 3, "waves must be 2, 4, 8 or 16");
+    GPUTILS_ARG(per_cu >= 1 && per_cu <= 64, 4, "per_cu must be in [1, 64]");
+    // the device tables change: no execution of the plan may be pending
+    GPUTILS_HIP(hipDeviceSynchronize());
+    auto &bk = plan->buckets[(size_t)bucket];
+    ragged_set_shape(plan, bk, waves, per_cu);
+    return ragged_upload_bucket(plan, bk);
+}
+
+int gputils_gels_ragged_plan_tune(gputils_gels_plan_t plan, const double *A, const double *B, int refine_steps,
+                                  void *stream) {
+    GPUTILS_ARG(plan != nullptr, 1, "plan is NULL");
+    GPUTILS_ARG(refine_steps >= 0 && refine_steps <= 16, 4, "refine_steps must be in [0, 16]");
+    if (plan->count == 0) return GPUTILS_OK;
+    GPUTILS_ARG(A != nullptr, 2, "A is NULL");
+    GPUTILS_ARG(B != nullptr, 3, "B is NULL");
+    hipStream_t st = as_stream(stream);
+    GPUTILS_HIP(hipDeviceSynchronize());
+    // candidates run on a copy of B (refreshed before every run) so the
+    // caller's right-hand sides stay as they are
+    double *scratch = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipError_t e = hipMalloc(&scratch, sizeof(double) * (size_t)plan->extentB);
+    if (e == hipSuccess) e = hipEventCreate(&ev0);
+    if (e == hipSuccess) e = hipEventCreate(&ev1);
+    int rc = e == hipSuccess ? GPUTILS_OK : hip_fail(e, "tune set-up");
+    constexpr int kRepeats = 3;
+    for (size_t bi = 0; rc == GPUTILS_OK && bi < plan->buckets.size(); ++bi) {
+        auto &bk = plan->buckets[bi];
+        int best_w = bk.waves, best_pc = bk.per_cu;
+        float best_ms = -1.0f;
+        std::vector<std::pair<int, int>> seen;  // (waves, grid) already timed
+        for (int waves : {2, 4, 8, 16}) {
+            const int cap = ragged_max_per_cu(bk.maxe, waves);
+            for (int pc = 1; pc <= cap && rc == GPUTILS_OK; ++pc) {
+                ragged_set_shape(plan, bk, waves, pc);
+                if (std::find(seen.begin(), seen.end(), std::make_pair(waves, bk.grid)) != seen.end()) continue;
+                seen.push_back({waves, bk.grid});
+                rc = ragged_upload_bucket(plan, bk);
+                float t_min = -1.0f;
+                for (int r = 0; rc == GPUTILS_OK && r <= kRepeats; ++r) {  // run 0 warms up
+                    e = hipMemcpyAsync(scratch, B, sizeof(double) * (size_t)plan->extentB, hipMemcpyDeviceToDevice, st);
+                    if (e == hipSuccess) e = hipEventRecord(ev0, st);
+                    if (e != hipSuccess) {
+                        rc = hip_fail(e, "tune");
+                        break;
+                    }
+                    rc = qr::launch_ragged(bk.maxe, bk.waves, bk.grid, st, plan->d_items, plan->d_wstart + bk.wbase, A,
+                                           scratch, refine_steps);
+                    if (rc != GPUTILS_OK) break;
+                    e = hipEventRecord(ev1, st);
+                    if (e == hipSuccess) e = hipEventSynchronize(ev1);
+                    float ms = 0.0f;
+                    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev0, ev1);
+                    if (e != hipSuccess) {
+                        rc = hip_fail(e, "tune timing");
+                        break;
+                    }
+                    if (r > 0 && (t_min < 0.0f || ms < t_min)) t_min = ms;
+                }
+                if (rc == GPUTILS_OK && (best_ms < 0.0f || t_min < best_ms)) {
+                    best_ms = t_min;
+                    best_w = waves;
+                    best_pc = pc;
+                }
+            }
+        }
+        ragged_set_shape(plan, bk, best_w, best_pc);
+        if (rc == GPUTILS_OK) rc = ragged_upload_bucket(plan, bk);
+    }
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (scratch) {
+        (void)hipStreamSynchronize(st);
+        (void)hipFree(scratch);
+    }
+    return rc;
+}
+
 int gputils_dgels_ragged(gputils_gels_plan_t plan, const double *A, double *B, int refine_steps,
                          void *stream) {
     GPUTILS_ARG(plan != nullptr, 1, "plan is NULL");
@@ -569,19 +732,8 @@ int gputils_dgels_ragged(gputils_gels_plan_t plan, const double *A, double *B, i
     GPUTILS_ARG(B != nullptr, 3, "B is NULL");
     hipStream_t st = as_stream(stream);
     for (const auto &bk : plan->buckets) {
-        const qr::RaggedItem *it = plan->d_items;  // wstart holds absolute item offsets
-        const int *ws = plan->d_wstart + bk.wbase;
-        if (bk.maxe == 2048)
-            hipLaunchKernelGGL((qr::gels_ragged_kernel<2048>), dim3(bk.grid), dim3(qr::kThreads), 0, st, it, ws, A, B, refine_steps);
-        else if (bk.maxe == 4096)
-            hipLaunchKernelGGL((qr::gels_ragged_kernel<4096>), dim3(bk.grid), dim3(qr::kThreads), 0, st, it, ws, A, B, refine_steps);
-        else if (bk.maxe == 8192)
-            hipLaunchKernelGGL((qr::gels_ragged_kernel<8192>), dim3(bk.grid), dim3(qr::kThreads), 0, st, it, ws, A, B, refine_steps);
-        else if (bk.maxe == 16384)
-            hipLaunchKernelGGL((qr::gels_ragged_kernel<16384>), dim3(bk.grid), dim3(qr::kThreads), 0, st, it, ws, A, B, refine_steps);
-        else
-            hipLaunchKernelGGL((qr::gels_ragged_kernel<32768>), dim3(bk.grid), dim3(qr::kThreads), 0, st, it, ws, A, B, refine_steps);
-        int rc = launch_check("gels_ragged_kernel");
+        int rc = qr::launch_ragged(bk.maxe, bk.waves, bk.grid, st, plan->d_items, plan->d_wstart + bk.wbase, A, B,
+                                   refine_steps);
         if (rc) return rc;
     }
     return GPUTILS_OK;

@@ -207,6 +207,32 @@ int gputils_gels_ragged_plan_create(gputils_gels_plan_t *plan, int64_t count, co
 int gputils_gels_ragged_plan_destroy(gputils_gels_plan_t plan);
 int gputils_dgels_ragged(gputils_gels_plan_t plan, const double *A, double *B, int refine_steps,
                          void *stream);
+/* Launch shape per bucket.  A plan has one bucket per LDS size class in use
+ * (FP32 m*n <= 2048, 4096, 8192, 16384, 32768; largest first), one launch
+ * each.  Its shape: wavefronts per workgroup (2, 4, 8 or 16) and workgroups per
+ * CU (the grid is min(systems in the bucket, CUs * per_cu)); the systems
+ * are assigned to the grid's workgroups longest-processing-time first.  A
+ * new plan uses 8 wavefronts and as many workgroups per CU as fit (LDS,
+ * 32 wavefronts per CU).
+ * The shape changes the speed only: every shape gives bit-identical
+ * results.
+ *
+ * describe: writes {maxe, systems, waves, per_cu, grid} for the first
+ * min(capacity, buckets) buckets to desc (5 int32 each) and the bucket count
+ * to *nbuckets (if not NULL).
+ * configure: sets one bucket's shape.
+ * tune: the autotuner.  For every bucket it times each shape (waves 2/4/8/16
+ * times every per_cu the CU can hold) on the caller's A and a device copy
+ * of B (B itself is not written), with refine_steps refinements, on
+ * stream, and keeps the fastest.  It allocates the copy of B and so is a
+ * plan-time call like plan_create.
+ * configure and tune synchronise the device first and rewrite the plan's
+ * device tables: no execution of the plan may be pending or run
+ * concurrently. */
+int gputils_gels_ragged_plan_describe(gputils_gels_plan_t plan, int32_t *desc, int capacity, int *nbuckets);
+int gputils_gels_ragged_plan_configure(gputils_gels_plan_t plan, int bucket, int waves, int per_cu);
+int gputils_gels_ragged_plan_tune(gputils_gels_plan_t plan, const double *A, const double *B, int refine_steps,
+                                  void *stream);
 
 /* ------------------------------------------------------------ graphs --- */
 /* Capture the work enqueued on a (non-null) stream into an executable HIP

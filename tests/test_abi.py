@@ -54,6 +54,9 @@ def _msg():
     ("gputils_dgemm_strided_batched", (2, 2, 2, 1.0, None, 2, 4, None, 2, 4, 0.0, None, 1, 4, 1, None), -13),
     ("gputils_dnrm2", (-5, None, None, None, None), -1),
     ("gputils_stranspose_batched", (3, 2, None, 2, 6, None, 2, 6, 1, None), -4),
+    ("gputils_gels_ragged_plan_configure", (None, 0, 4, 1), -1),
+    ("gputils_gels_ragged_plan_describe", (None, None, 0, None), -1),
+    ("gputils_gels_ragged_plan_tune", (None, None, None, 1, None), -1),
 ])
 def test_invalid_arguments_are_rejected(fn, args, code):
     assert getattr(_lib.lib(), fn)(*args) == code
@@ -67,6 +70,26 @@ def test_empty_problems_are_noops():
     assert L.gputils_sgemm_strided_batched(0, 3, 3, 1.0, None, 1, 0, None, 3, 9, 0.0, None, 1, 0, 4, 0, None) == 0
     assert L.gputils_dposv_mixed_batched(8, 0, None, 8, 64, None, 8, 0, None, 8, 0, 1e-12, 3, None, None, 9,
                                           None) == 0
+
+
+def test_empty_ragged_plan_shape_calls():
+    """A plan of zero systems has no buckets: describe reports none, tuning
+    is a no-op and configuring a bucket is out of range (no device memory
+    is touched, so this runs without a GPU)."""
+    import ctypes
+    L = _lib.lib()
+    plan = ctypes.c_void_p()
+    assert L.gputils_gels_ragged_plan_create(ctypes.byref(plan), 0, None, None, None, None) == 0
+    try:
+        nb = ctypes.c_int(-1)
+        assert L.gputils_gels_ragged_plan_describe(plan, None, 0, ctypes.byref(nb)) == 0
+        assert nb.value == 0
+        assert L.gputils_gels_ragged_plan_tune(plan, None, None, 1, None) == 0
+        assert L.gputils_gels_ragged_plan_configure(plan, 0, 4, 1) == -2
+        assert "invalid argument 2" in _msg()
+        assert L.gputils_gels_ragged_plan_configure(plan, 0, 3, 1) == -2  # the bucket is checked first
+    finally:
+        assert L.gputils_gels_ragged_plan_destroy(plan) == 0
 
 
 def test_no_oracle_in_the_product():

@@ -170,6 +170,81 @@ def test_ragged_least_squares_unrefined_is_fp32_level(oracle):
     assert max(e for e, *_ in errs) > 1e-9
 
 
+def _ragged_solve(plan, tA, Bf, steps):
+    tB = dev(Bf)
+    plan(tA, tB, refine_steps=steps)
+    return tB.cpu().numpy()
+
+
+def _bucket_sizes(ms, ns):
+    """Systems per LDS size class (FP32 m*n <= 2048 .. 32768), largest first."""
+    e = ms.astype(np.int64) * ns
+    out = {}
+    for x in e:
+        cls = next(c for c in (2048, 4096, 8192, 16384, 32768) if x <= c)
+        out[cls] = out.get(cls, 0) + 1
+    return [(c, out[c]) for c in sorted(out, reverse=True)]
+
+
+def test_ragged_plan_buckets_and_default_shape():
+    from gputils_amd import workloads
+    ms, ns, offA, offB, Af, Bf = workloads.ls_ragged()
+    plan = api.RaggedLeastSquares(ms, ns, offA, offB)
+    desc = plan.describe()
+    assert [(d["maxe"], d["systems"]) for d in desc] == _bucket_sizes(ms, ns)
+    for d in desc:
+        assert d["waves"] == 8 and d["per_cu"] >= 1 and 1 <= d["grid"] <= d["systems"]
+
+
+def test_ragged_every_launch_shape_gives_identical_results():
+    """The launch shape (wavefronts per workgroup, workgroups per CU and so
+    the assignment of systems to workgroups) changes no system's arithmetic:
+    every shape reproduces the default shape's solutions bit for bit, also
+    where a workgroup solves several systems in turn (per_cu = 1)."""
+    from gputils_amd import workloads
+    ms, ns, offA, offB, Af, Bf = workloads.ls_ragged()
+    plan = api.RaggedLeastSquares(ms, ns, offA, offB)
+    tA = dev(Af)
+    ref = _ragged_solve(plan, tA, Bf, 1)
+    nb = len(plan.describe())
+    for waves in (2, 4, 8, 16):
+        for per_cu in (1, 3):
+            for b in range(nb):
+                plan.configure(b, waves, per_cu)
+            for d in plan.describe():
+                assert d["waves"] == waves and d["per_cu"] == per_cu and 1 <= d["grid"] <= d["systems"]
+            got = _ragged_solve(plan, tA, Bf, 1)
+            assert np.array_equal(got, ref), (waves, per_cu)
+
+
+def test_ragged_autotune_keeps_b_and_results():
+    """The autotuner times the shapes on a copy of B (B is not written) and
+    leaves a valid shape per bucket that solves bit-identically."""
+    from gputils_amd import workloads
+    ms, ns, offA, offB, Af, Bf = workloads.ls_ragged()
+    plan = api.RaggedLeastSquares(ms, ns, offA, offB)
+    tA, tB = dev(Af), dev(Bf)
+    ref = _ragged_solve(plan, tA, Bf, 1)
+    desc = plan.tune(tA, tB, refine_steps=1)
+    assert np.array_equal(tB.cpu().numpy(), Bf)
+    assert [(d["maxe"], d["systems"]) for d in desc] == _bucket_sizes(ms, ns)
+    for d in desc:
+        assert d["waves"] in (2, 4, 8, 16) and d["per_cu"] >= 1 and 1 <= d["grid"] <= d["systems"]
+    assert np.array_equal(_ragged_solve(plan, tA, Bf, 1), ref)
+
+
+def test_ragged_plan_configure_rejects_bad_shapes():
+    from gputils_amd import workloads
+    ms, ns, offA, offB, Af, Bf = workloads.ls_ragged(count=64)
+    plan = api.RaggedLeastSquares(ms, ns, offA, offB)
+    with pytest.raises(api.GputilsError):
+        plan.configure(0, 3, 1)       # waves must be 2, 4, 8 or 16
+    with pytest.raises(api.GputilsError):
+        plan.configure(0, 4, 0)       # per_cu >= 1
+    with pytest.raises(api.GputilsError):
+        plan.configure(len(plan.describe()), 4, 1)  # no such bucket
+
+
 def test_ragged_plan_rejects_bad_sizes():
     with pytest.raises(api.GputilsError):
         api.RaggedLeastSquares([10], [20], [0], [0])   # n > m
