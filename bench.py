@@ -51,6 +51,9 @@ def _args(argv=None):
                     choices=["cholesky_mixed", "gemm", "cholesky_small", "ls_ragged", "svd_nullspace",
                              "copy_calibration"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ragged-shapes", default=None,
+                    help="ls_ragged: launch shapes per bucket as WAVES:PER_CU,... (largest bucket first) "
+                         "applied instead of autotuning; the default-shape timing is then skipped")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison runs")
     return ap.parse_args(argv)
 
@@ -416,13 +419,22 @@ def bench_ls_ragged(a):
     # steps (the API default) reported beside it.  The plan's launch shapes
     # are autotuned once on these inputs (plan-time work, like building the
     # plan, outside the timed region); the default shapes are timed first.
-    ev_default, _ = run(1)
     default_shapes = plan.describe()
-    torch.cuda.synchronize()
-    t_tune = time.perf_counter()
-    shapes = plan.tune(dA, dB0, refine_steps=1)
-    t_tune = time.perf_counter() - t_tune
-    print(f"[bench] ls_ragged autotune: {t_tune:.2f}s, shapes {shapes}", file=sys.stderr, flush=True)
+    if a.ragged_shapes:
+        given = [tuple(int(v) for v in x.split(":")) for x in a.ragged_shapes.split(",")]
+        if len(given) != len(default_shapes):
+            raise SystemExit(f"--ragged-shapes needs {len(default_shapes)} shapes, got {len(given)}")
+        for b, (waves, per_cu) in enumerate(given):
+            plan.configure(b, waves, per_cu)
+        ev_default, t_tune, shapes = None, None, plan.describe()
+        print(f"[bench] ls_ragged given shapes {shapes}", file=sys.stderr, flush=True)
+    else:
+        ev_default, _ = run(1)
+        torch.cuda.synchronize()
+        t_tune = time.perf_counter()
+        shapes = plan.tune(dA, dB0, refine_steps=1)
+        t_tune = time.perf_counter() - t_tune
+        print(f"[bench] ls_ragged autotune: {t_tune:.2f}s, shapes {shapes}", file=sys.stderr, flush=True)
     ev, errs = run(1)
     ev2, errs2 = run(2)
     alg = int(8 * (np.sum(ms.astype(np.int64) * ns) + np.sum(ms) + np.sum(ns)))
@@ -448,9 +460,10 @@ def bench_ls_ragged(a):
                                 "median_rel_err_vs_dgels_sample64": float(np.median(errs))},
                  two_refinement_steps={"event_ms": ev2, "systems_per_s": count / (ev2 * 1e-3),
                                        "max_rel_err_vs_dgels_sample64": float(max(errs2))},
-                 autotune={"tune_seconds": t_tune, "default_shape_event_ms": ev_default,
+                 autotune={"tuned": not a.ragged_shapes, "tune_seconds": t_tune,
+                           "default_shape_event_ms": ev_default,
                            "default_shapes": [[d["maxe"], d["waves"], d["per_cu"], d["grid"]] for d in default_shapes],
-                           "tuned_shapes": [[d["maxe"], d["waves"], d["per_cu"], d["grid"]] for d in shapes],
+                           "shapes": [[d["maxe"], d["waves"], d["per_cu"], d["grid"]] for d in shapes],
                            "shape_fields": ["maxe", "waves", "per_cu", "grid"]},
                  fp_rate_tflops=flops / (ev * 1e-3) / 1e12)
 
