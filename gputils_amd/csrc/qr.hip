@@ -160,19 +160,6 @@ struct RaggedItem {
     int64_t offA, offB;
 };
 
-// Solve R^T y = c in place (forward substitution), one wavefront.
-template <typename T>
-__device__ __forceinline__ void trsv_upper_trans(const T *A, int64_t lda, int n, T *c) {
-    const int lane = threadIdx.x & 63;
-    for (int k = 0; k < n; ++k) {
-        const T ck = c[k] / A[k + (int64_t)k * lda];
-        for (int i = k + 1 + lane; i < n; i += kWave) c[i] -= A[k + (int64_t)i * lda] * ck;
-        wave_sync();
-        if (lane == 0) c[k] = ck;
-        wave_sync();
-    }
-}
-
 // FP64 r_i = base_i - sum_k A[i][k] x[k] for the rows i of this thread
 // (row-major over the NT threads, coalesced over rows); the loads of eight
 // columns are issued together so that they are in flight at once.
@@ -192,6 +179,94 @@ __device__ __forceinline__ void rows_minus_ax(const double *__restrict__ a, int 
         for (; k < n; ++k) r = fma(-a[i + (int64_t)k * m], x[k], r);
         out[i] = r;
     }
+}
+
+// The one-wavefront vector pieces of the ragged solver keep their vector
+// in registers, so each step costs one batch of LDS reads of the factors
+// plus one reduction or one pivot broadcast instead of a chain of LDS
+// round trips.
+
+// c <- Q^T c (trans) or Q c for one m-vector in LDS (m <= 512), one
+// wavefront; Q from the n reflectors stored below the diagonal of S (ld m)
+// and tau.  Rows lane + 64 u of c are held in registers; reflector j is
+// applied as w = v . c, c -= (tau_j w) v with v = [0 .. 0, 1, S[j+1:m, j]]
+// (its eight loads per lane issued together; reading them a step ahead
+// would cost the kernel 28 more VGPRs and a wavefront per SIMD).
+__device__ __forceinline__ void apply_q_regs(const float *S, int m, int n, const float *tau, float *c, bool trans) {
+    constexpr int R = 8;
+    const int lane = threadIdx.x & 63;
+    const int ru = (m + 63) >> 6;
+    float cr[R], vc[R];
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+        const int i = lane + 64 * u;
+        cr[u] = (u < ru && i < m) ? c[i] : 0.0f;
+    }
+    // rows past m of column j lie inside S (j m + 512 <= m n + 512), so the
+    // loads need no per-lane guard; the select masks them
+    auto load_v = [&](int j, float *vr) {
+        const float *col = S + j * m;
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const int i = lane + 64 * u;
+            const float e = (u < ru) ? col[i] : 0.0f;
+            vr[u] = (i > j && i < m) ? e : ((i == j) ? 1.0f : 0.0f);
+        }
+    };
+    const int dj = trans ? 1 : -1;
+    int j = trans ? 0 : n - 1;
+#pragma unroll 1
+    for (int t = 0; t < n; ++t, j += dj) {
+        load_v(j, vc);
+        float w = 0.0f;
+#pragma unroll
+        for (int u = 0; u < R; ++u)
+            if (u < ru) w = fmaf(vc[u], cr[u], w);
+        w = wave_sum(w);
+        const float tw = tau[j] * w;
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            if (u < ru) cr[u] = fmaf(-tw, vc[u], cr[u]);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+        const int i = lane + 64 * u;
+        if (u < ru && i < m) c[i] = cr[u];
+    }
+}
+
+// Solve R y = c in place (R the n x n upper triangle of S, ld m; c in LDS,
+// n <= 64), one wavefront: lane i holds c_i, column k - 1 of R is read while
+// step k runs.
+__device__ __forceinline__ void trsv_upper_regs(const float *S, int m, int n, float *c) {
+    const int lane = threadIdx.x & 63;
+    float cv = (lane < n) ? c[lane] : 0.0f;
+    float rk = (lane < n) ? S[lane + (int64_t)(n - 1) * m] : 0.0f;
+#pragma unroll 1
+    for (int k = n - 1; k >= 0; --k) {
+        const float rn = (lane < k) ? S[lane + (int64_t)(k - 1) * m] : 0.0f;  // rows 0..k-1 of column k-1
+        const float ck = readlane(cv, k) / readlane(rk, k);
+        cv = (lane == k) ? ck : ((lane < k) ? fmaf(-rk, ck, cv) : cv);
+        rk = rn;
+    }
+    if (lane < n) c[lane] = cv;
+}
+
+// Solve R^T y = c in place (forward substitution), one wavefront: lane i
+// holds c_i, row k + 1 of R is read while step k runs.
+__device__ __forceinline__ void trsv_upper_trans_regs(const float *S, int m, int n, float *c) {
+    const int lane = threadIdx.x & 63;
+    float cv = (lane < n) ? c[lane] : 0.0f;
+    float rk = (lane < n) ? S[(int64_t)lane * m] : 0.0f;  // row 0
+#pragma unroll 1
+    for (int k = 0; k < n; ++k) {
+        const float rn = (lane > k && lane < n) ? S[(k + 1) + (int64_t)lane * m] : 0.0f;  // row k+1, columns > k
+        const float ck = readlane(cv, k) / readlane(rk, k);
+        cv = (lane == k) ? ck : ((lane > k) ? fmaf(-rk, ck, cv) : cv);
+        rk = rn;
+    }
+    if (lane < n) c[lane] = cv;
 }
 
 // One bucket of the ragged plan: workgroup w solves the items
@@ -252,7 +327,7 @@ __global__ __launch_bounds__(NW * kWave) void gels_ragged_kernel(const RaggedIte
         }
         __syncthreads();
         hqr_factor<float, double, float, NW>(S, m, m, n, tau, s_tau, n + 1);  // FP64 norms, FP32 dots
-        if (wave == 0) trsv_upper<float>(S, m, n, qb);  // x0 = R^{-1} (Q^T b)[0:n]
+        if (wave == 0) trsv_upper_regs(S, m, n, qb);  // x0 = R^{-1} (Q^T b)[0:n]
         __syncthreads();
         for (int i = threadIdx.x; i < n; i += NT) xd[i] = (double)qb[i];
         __syncthreads();
@@ -294,8 +369,8 @@ __global__ __launch_bounds__(NW * kWave) void gels_ragged_kernel(const RaggedIte
             __syncthreads();
             // the two independent serial pieces of each half run on two
             // wavefronts at once
-            if (wave == 0) trsv_upper_trans<float>(S, m, n, h);                 // h = R^{-T} g
-            if (wave == 1) apply_q_vec<float, float>(S, m, m, n, tau, c, true);  // c = Q^T f
+            if (wave == 0) trsv_upper_trans_regs(S, m, n, h);       // h = R^{-T} g
+            if (wave == 1) apply_q_regs(S, m, n, tau, c, true);   // c = Q^T f
             __syncthreads();
             for (int i = threadIdx.x; i < n; i += NT) {
                 dx[i] = c[i] - h[i];
@@ -303,10 +378,10 @@ __global__ __launch_bounds__(NW * kWave) void gels_ragged_kernel(const RaggedIte
             }
             __syncthreads();
             if (wave == 0) {
-                trsv_upper<float>(S, m, n, dx);                               // dx = R^{-1} (d1 - h)
+                trsv_upper_regs(S, m, n, dx);                                 // dx = R^{-1} (d1 - h)
                 for (int i = lane; i < n; i += kWave) xd[i] += (double)dx[i];
             }
-            if (wave == 1) apply_q_vec<float, float>(S, m, m, n, tau, c, false);  // c = dr = Q [h; d2]
+            if (wave == 1) apply_q_regs(S, m, n, tau, c, false);  // c = dr = Q [h; d2]
             __syncthreads();
             for (int i = threadIdx.x; i < m; i += NT) rd[i] += (double)c[i];
         }
