@@ -52,8 +52,8 @@ def _args(argv=None):
                              "copy_calibration"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ragged-shapes", default=None,
-                    help="ls_ragged: launch shapes per bucket as WAVES:PER_CU,... (largest bucket first) "
-                         "applied instead of autotuning; the default-shape timing is then skipped")
+                    help="ls_ragged: launch shapes per bucket as METHOD:WAVES:PER_CU,... (largest bucket "
+                         "first) applied instead of autotuning; the default-shape timing is then skipped")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison runs")
     return ap.parse_args(argv)
 
@@ -424,8 +424,8 @@ def bench_ls_ragged(a):
         given = [tuple(int(v) for v in x.split(":")) for x in a.ragged_shapes.split(",")]
         if len(given) != len(default_shapes):
             raise SystemExit(f"--ragged-shapes needs {len(default_shapes)} shapes, got {len(given)}")
-        for b, (waves, per_cu) in enumerate(given):
-            plan.configure(b, waves, per_cu)
+        for b, (method, waves, per_cu) in enumerate(given):
+            plan.configure(b, method, waves, per_cu)
         ev_default, t_tune, shapes = None, None, plan.describe()
         print(f"[bench] ls_ragged given shapes {shapes}", file=sys.stderr, flush=True)
     else:
@@ -462,9 +462,10 @@ def bench_ls_ragged(a):
                                        "max_rel_err_vs_dgels_sample64": float(max(errs2))},
                  autotune={"tuned": not a.ragged_shapes, "tune_seconds": t_tune,
                            "default_shape_event_ms": ev_default,
-                           "default_shapes": [[d["maxe"], d["waves"], d["per_cu"], d["grid"]] for d in default_shapes],
-                           "shapes": [[d["maxe"], d["waves"], d["per_cu"], d["grid"]] for d in shapes],
-                           "shape_fields": ["maxe", "waves", "per_cu", "grid"]},
+                           "default_shapes": [[d["maxe"], d["method"], d["waves"], d["per_cu"], d["grid"]]
+                                              for d in default_shapes],
+                           "shapes": [[d["maxe"], d["method"], d["waves"], d["per_cu"], d["grid"]] for d in shapes],
+                           "shape_fields": ["maxe", "method", "waves", "per_cu", "grid"]},
                  fp_rate_tflops=flops / (ev * 1e-3) / 1e12)
 
 

@@ -280,17 +280,18 @@ class RaggedLeastSquares:
 
     def describe(self):
         """Launch shape of every bucket: a list of dicts with keys maxe (LDS
-        size class, FP32 elements), systems, waves, per_cu, grid."""
+        size class, FP32 elements), systems, method, waves, per_cu, grid."""
         nb = ctypes.c_int(0)
         check(lib().gputils_gels_ragged_plan_describe(self._plan, None, 0, ctypes.byref(nb)), "plan_describe")
-        desc = (ctypes.c_int32 * (5 * max(nb.value, 1)))()
+        desc = (ctypes.c_int32 * (6 * max(nb.value, 1)))()
         check(lib().gputils_gels_ragged_plan_describe(self._plan, desc, nb.value, None), "plan_describe")
-        keys = ("maxe", "systems", "waves", "per_cu", "grid")
-        return [dict(zip(keys, desc[5 * i:5 * i + 5])) for i in range(nb.value)]
+        keys = ("maxe", "systems", "method", "waves", "per_cu", "grid")
+        return [dict(zip(keys, desc[6 * i:6 * i + 6])) for i in range(nb.value)]
 
-    def configure(self, bucket: int, waves: int, per_cu: int):
-        """Set one bucket's launch shape (waves 2, 4, 8 or 16; workgroups per CU)."""
-        check(lib().gputils_gels_ragged_plan_configure(self._plan, bucket, waves, per_cu), "plan_configure")
+    def configure(self, bucket: int, method: int, waves: int, per_cu: int):
+        """Set one bucket's launch shape: method 0 (column steps in LDS,
+        waves 2/4/8/16) or 1 (register panels, waves 8/16), workgroups per CU."""
+        check(lib().gputils_gels_ragged_plan_configure(self._plan, bucket, method, waves, per_cu), "plan_configure")
 
     def tune(self, A_flat: torch.Tensor, B_flat: torch.Tensor, refine_steps: int = 1):
         """Autotune every bucket's launch shape on these inputs (B_flat is not

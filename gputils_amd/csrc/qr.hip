@@ -181,6 +181,173 @@ __device__ __forceinline__ void rows_minus_ax(const double *__restrict__ a, int 
     }
 }
 
+// Panel-blocked, register-resident FP32 Householder QR of the m x (n + 1)
+// matrix [A b] for the ragged solver (A FP64 column-major with ld m, b an
+// FP64 m-vector; m <= 512, n <= 64), dlarfg conventions, column norms in
+// FP64 and reflector dot products in FP32.
+//
+// Columns are grouped in panels of NB = 64 / NW; panel P is held by
+// wavefront P % NW in registers (rows lane + 64 u, u < 8) and b by the
+// wavefront after the last panel's.  Iteration p: every wavefront applies
+// the reflectors of panel p (read from S, where their owner stored them) to
+// its columns right of the panel, one reflector at a time: w = v . x,
+// x -= (tau w) v, with v = [0 .. 0, 1, v_{j+1} .. v_{m-1}] and zeros past
+// m; then the owner of panel p + 1 factorises it in its registers alone
+// (no barrier, no LDS traffic inside the panel) and stores it to S.  One
+// barrier per panel.  Each column's arithmetic is the same for every NW.
+// At the end S (column-major, ld m) holds R on and above the diagonal and
+// v below it, column n holds Q^T b, and tau[0:n] the scalars.
+template <int NW>
+__device__ __forceinline__ void hqr_panels(const double *__restrict__ a, const double *__restrict__ bcol, float *S,
+                                           int m, int n, float *tau) {
+    constexpr int NB = 64 / NW;  // columns per panel; one panel per wavefront
+    constexpr int CM = NB + 1;   // register columns: the panel and maybe b
+    constexpr int R = 8;         // row groups (m <= 512)
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ru = (m + 63) >> 6;
+    const int np = (n + NB - 1) / NB;
+    const bool own_b = wave == (np % NW);
+    // column of register slot q (q < NB: panel `wave`; q == NB: b)
+    auto col_of = [&](int q) { return (q < NB) ? wave * NB + q : n; };
+    auto valid = [&](int q) { return (q < NB) ? (wave * NB + q < n) : own_b; };
+    // every lane loads a valid element (row and column clamped into the
+    // system) so that a column's eight loads are unconditional and in
+    // flight together; the padding is zeroed afterwards.  The compiler
+    // barrier per column keeps the doubles in flight to one column's worth
+    // of registers.
+    float x[CM][R];
+#pragma unroll
+    for (int q = 0; q < CM; ++q) {
+        const bool ok = valid(q);
+        const double *src = (q < NB) ? a + (int64_t)(ok ? col_of(q) : 0) * m : bcol;
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const int i = lane + 64 * u;
+            x[q][u] = (float)src[i < m ? i : m - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const int i = lane + 64 * u;
+            if (!(ok && i < m)) x[q][u] = 0.0f;
+        }
+        asm volatile("" ::: "memory");
+    }
+    // reflector of column cc from its registers; col becomes [R part;
+    // beta; v], vn the full reflector vector; returns tau
+    auto reflect = [&](float (&col)[R], int cc, float (&vn)[R]) -> float {
+        const int uc = cc >> 6;
+        float asel = col[0];
+#pragma unroll
+        for (int u = 1; u < R; ++u)
+            if (u == uc) asel = col[u];
+        const double alpha = (double)readlane(asel, cc & 63);
+        double sq = 0.0;  // rows past m hold zeros
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const int i = lane + 64 * u;
+            const double e = (double)col[u];
+            sq += (i > cc) ? e * e : 0.0;
+        }
+        sq = wave_sum(sq);
+        float t1 = 0.0f;
+        if (sq != 0.0) {
+            const double norm = sqrt(alpha * alpha + sq);
+            const double beta = (alpha >= 0.0) ? -norm : norm;
+            t1 = (float)((beta - alpha) / beta);
+            const float scale = (float)(1.0 / (alpha - beta));
+#pragma unroll
+            for (int u = 0; u < R; ++u) {
+                const int i = lane + 64 * u;
+                col[u] = (i > cc) ? col[u] * scale : ((i == cc) ? (float)beta : col[u]);
+                vn[u] = (i > cc) ? col[u] : ((i == cc) ? 1.0f : 0.0f);
+            }
+        } else {  // H = I (dlarfg with x = 0): the column stays as it is
+#pragma unroll
+            for (int u = 0; u < R; ++u) vn[u] = (lane + 64 * u == cc) ? 1.0f : 0.0f;
+        }
+        return t1;
+    };
+    // x_q -= (t v . x_q) v for the slots with sel(q)
+    auto apply = [&](const float (&vr)[R], float t, auto sel) {
+        float w[CM];
+#pragma unroll
+        for (int q = 0; q < CM; ++q) {
+            w[q] = 0.0f;
+            if (sel(q)) {
+#pragma unroll
+                for (int u = 0; u < R; ++u)
+                    if (u < ru) w[q] = fmaf(vr[u], x[q][u], w[q]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < CM; ++q)
+            if (sel(q)) w[q] = wave_sum(w[q]);
+#pragma unroll
+        for (int q = 0; q < CM; ++q) {
+            if (sel(q)) {
+                const float tw = t * w[q];
+#pragma unroll
+                for (int u = 0; u < R; ++u)
+                    if (u < ru) x[q][u] = fmaf(-tw, vr[u], x[q][u]);
+            }
+        }
+    };
+    // factorise this wavefront's panel (columns wave * NB ..) and store it
+    auto factor_panel = [&]() {
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            const int c = wave * NB + k;
+            if (c < n) {
+                float vn[R];
+                const float t1 = reflect(x[k], c, vn);
+                if (lane == 0) tau[c] = t1;
+                apply(vn, t1, [&](int q) { return q > k && q < NB && wave * NB + q < n; });
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            const int c = wave * NB + k;
+            if (c < n) {
+#pragma unroll
+                for (int u = 0; u < R; ++u) {
+                    const int i = lane + 64 * u;
+                    if (i < m) S[i + c * m] = x[k][u];
+                }
+            }
+        }
+    };
+    if (wave == 0) factor_panel();
+    __syncthreads();
+    for (int p = 0; p < np; ++p) {
+        const int j0 = p * NB, j1 = min(j0 + NB, n);
+        // panel p's reflectors on every column right of it (rows past m of
+        // a column of S lie inside S: j m + 512 <= m n + 512)
+        if (wave > p || own_b) {
+            for (int j = j0; j < j1; ++j) {
+                const float *colv = S + j * m;
+                float vr[R];
+#pragma unroll
+                for (int u = 0; u < R; ++u) {
+                    const int i = lane + 64 * u;
+                    const float e = (u < ru) ? colv[i] : 0.0f;
+                    vr[u] = (i > j && i < m) ? e : ((i == j) ? 1.0f : 0.0f);
+                }
+                apply(vr, tau[j], [&](int q) { return valid(q) && col_of(q) >= j1; });
+            }
+        }
+        if (wave == p + 1 && p + 1 < np) factor_panel();
+        __syncthreads();
+    }
+    if (own_b) {
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const int i = lane + 64 * u;
+            if (i < m) S[i + n * m] = x[NB][u];
+        }
+    }
+    __syncthreads();
+}
+
 // The one-wavefront vector pieces of the ragged solver keep their vector
 // in registers, so each step costs one batch of LDS reads of the factors
 // plus one reduction or one pivot broadcast instead of a chain of LDS
@@ -271,8 +438,8 @@ __device__ __forceinline__ void trsv_upper_trans_regs(const float *S, int m, int
 
 // One bucket of the ragged plan: workgroup w solves the items
 // wstart[w] .. wstart[w+1]-1, a range the plan balanced by estimated cost
-// (longest-processing-time assignment).  Per system: FP64 A -> FP32 (LDS), FP32
-// Householder QR with FP64 column norms (b rides along as an extra column, so
+// (longest-processing-time assignment).  Per system: FP32 Householder QR
+// with FP64 column norms of FP64 A (b rides along as an extra column, so
 // the factorisation leaves Q^T b), x0 = R^{-1} Q^T b, r0 = b - A x0 in
 // FP64, then `steps` refinements of the augmented system
 // [I A; A^T 0][r; x] = [b; 0] (Bjorck 1967): f = b - r - A x, g = -A^T r
@@ -281,10 +448,15 @@ __device__ __forceinline__ void trsv_upper_trans_regs(const float *S, int m, int
 // accuracy also for inconsistent systems, where refining x alone stalls at
 // the FP32 level.  x overwrites b[0:n].  Global loads are batched (eight
 // per thread in flight) because one wavefront per SIMD leaves little else
-// to hide their latency.  NW wavefronts per workgroup (2, 4, 8 or 16; the plan
-// picks it per bucket); every system's arithmetic is the same for every NW
-// and every item-to-workgroup assignment, so the results are too.
-template <int MAXE, int NW>
+// to hide their latency.  METHOD selects the factorisation: 0 stages A in
+// LDS and factorises column by column (hqr_factor, lookahead column in
+// registers, one barrier per column; NW = 2, 4, 8 or 16), 1 keeps panels in
+// registers (hqr_panels, one barrier per panel; NW = 8 or 16).  The plan
+// picks METHOD, NW and the grid per bucket.  For one METHOD every system's
+// arithmetic is the same for every NW and every item-to-workgroup
+// assignment, so the results are too; the two methods round the FP32
+// factorisation differently.
+template <int MAXE, int NW, int METHOD>
 __global__ __launch_bounds__(NW * kWave) void gels_ragged_kernel(const RaggedItem *__restrict__ items,
                                                                const int *__restrict__ wstart,
                                                                const double *__restrict__ A, double *B,
@@ -307,7 +479,11 @@ __global__ __launch_bounds__(NW * kWave) void gels_ragged_kernel(const RaggedIte
         const double *a = A + w.offA;
         double *bb = B + w.offB;
         __syncthreads();
-        {
+        float *qb = S + m * n;  // b as column n: the factorisation turns it into Q^T b
+        if constexpr (METHOD == 1) {
+            for (int i = threadIdx.x; i < m; i += NT) bd[i] = bb[i];
+            hqr_panels<NW>(a, bb, S, m, n, tau);  // FP64 norms, FP32 dots
+        } else {
             const int mn = m * n;
             int e0 = threadIdx.x;
             for (; e0 + 7 * NT < mn; e0 += 8 * NT) {
@@ -318,15 +494,14 @@ __global__ __launch_bounds__(NW * kWave) void gels_ragged_kernel(const RaggedIte
                 for (int u = 0; u < 8; ++u) S[e0 + u * NT] = (float)v[u];
             }
             for (; e0 < mn; e0 += NT) S[e0] = (float)a[e0];
+            for (int i = threadIdx.x; i < m; i += NT) {
+                const double v = bb[i];
+                bd[i] = v;
+                qb[i] = (float)v;
+            }
+            __syncthreads();
+            hqr_factor<float, double, float, NW>(S, m, m, n, tau, s_tau, n + 1);  // FP64 norms, FP32 dots
         }
-        float *qb = S + m * n;  // b as column n: the factorisation turns it into Q^T b
-        for (int i = threadIdx.x; i < m; i += NT) {
-            const double v = bb[i];
-            bd[i] = v;
-            qb[i] = (float)v;
-        }
-        __syncthreads();
-        hqr_factor<float, double, float, NW>(S, m, m, n, tau, s_tau, n + 1);  // FP64 norms, FP32 dots
         if (wave == 0) trsv_upper_regs(S, m, n, qb);  // x0 = R^{-1} (Q^T b)[0:n]
         __syncthreads();
         for (int i = threadIdx.x; i < n; i += NT) xd[i] = (double)qb[i];
@@ -390,29 +565,38 @@ __global__ __launch_bounds__(NW * kWave) void gels_ragged_kernel(const RaggedIte
     }
 }
 
-// Launch one bucket of a ragged plan: the LDS capacity MAXE and the
-// wavefronts per workgroup pick the instantiation.
-template <int MAXE>
-void launch_ragged_waves(int waves, int grid, hipStream_t st, const RaggedItem *items, const int *wstart,
-                         const double *A, double *B, int steps) {
-    if (waves == 2)
-        hipLaunchKernelGGL((gels_ragged_kernel<MAXE, 2>), dim3(grid), dim3(2 * kWave), 0, st, items, wstart, A, B, steps);
-    else if (waves == 16)
-        hipLaunchKernelGGL((gels_ragged_kernel<MAXE, 16>), dim3(grid), dim3(16 * kWave), 0, st, items, wstart, A, B, steps);
-    else if (waves == 8)
-        hipLaunchKernelGGL((gels_ragged_kernel<MAXE, 8>), dim3(grid), dim3(8 * kWave), 0, st, items, wstart, A, B, steps);
-    else
-        hipLaunchKernelGGL((gels_ragged_kernel<MAXE, 4>), dim3(grid), dim3(4 * kWave), 0, st, items, wstart, A, B, steps);
+// Launch one bucket of a ragged plan: the LDS capacity MAXE, the
+// factorisation method and the wavefronts per workgroup pick the
+// instantiation.
+template <int MAXE, int METHOD, int NW>
+void launch_ragged_one(int grid, hipStream_t st, const RaggedItem *items, const int *wstart, const double *A,
+                       double *B, int steps) {
+    hipLaunchKernelGGL((gels_ragged_kernel<MAXE, NW, METHOD>), dim3(grid), dim3(NW * kWave), 0, st, items, wstart, A,
+                       B, steps);
 }
 
-int launch_ragged(int maxe, int waves, int grid, hipStream_t st, const RaggedItem *items, const int *wstart,
-                  const double *A, double *B, int steps) {
+template <int MAXE>
+void launch_ragged_maxe(int method, int waves, int grid, hipStream_t st, const RaggedItem *items, const int *wstart,
+                        const double *A, double *B, int steps) {
+    if (method == 1) {
+        if (waves == 16) launch_ragged_one<MAXE, 1, 16>(grid, st, items, wstart, A, B, steps);
+        else launch_ragged_one<MAXE, 1, 8>(grid, st, items, wstart, A, B, steps);
+    } else {
+        if (waves == 2) launch_ragged_one<MAXE, 0, 2>(grid, st, items, wstart, A, B, steps);
+        else if (waves == 4) launch_ragged_one<MAXE, 0, 4>(grid, st, items, wstart, A, B, steps);
+        else if (waves == 16) launch_ragged_one<MAXE, 0, 16>(grid, st, items, wstart, A, B, steps);
+        else launch_ragged_one<MAXE, 0, 8>(grid, st, items, wstart, A, B, steps);
+    }
+}
+
+int launch_ragged(int maxe, int method, int waves, int grid, hipStream_t st, const RaggedItem *items,
+                  const int *wstart, const double *A, double *B, int steps) {
     switch (maxe) {
-    case 2048: launch_ragged_waves<2048>(waves, grid, st, items, wstart, A, B, steps); break;
-    case 4096: launch_ragged_waves<4096>(waves, grid, st, items, wstart, A, B, steps); break;
-    case 8192: launch_ragged_waves<8192>(waves, grid, st, items, wstart, A, B, steps); break;
-    case 16384: launch_ragged_waves<16384>(waves, grid, st, items, wstart, A, B, steps); break;
-    default: launch_ragged_waves<32768>(waves, grid, st, items, wstart, A, B, steps); break;
+    case 2048: launch_ragged_maxe<2048>(method, waves, grid, st, items, wstart, A, B, steps); break;
+    case 4096: launch_ragged_maxe<4096>(method, waves, grid, st, items, wstart, A, B, steps); break;
+    case 8192: launch_ragged_maxe<8192>(method, waves, grid, st, items, wstart, A, B, steps); break;
+    case 16384: launch_ragged_maxe<16384>(method, waves, grid, st, items, wstart, A, B, steps); break;
+    default: launch_ragged_maxe<32768>(method, waves, grid, st, items, wstart, A, B, steps); break;
     }
     return launch_check("gels_ragged_kernel");
 }
@@ -471,7 +655,8 @@ struct gputils_gels_plan {
         int maxe;
         int begin, count;  // items begin .. begin + count - 1 of d_items
         int wbase;         // first of this bucket's count + 1 slots in d_wstart
-        int waves;         // wavefronts per workgroup: 2, 4, 8 or 16
+        int method;        // factorisation: 0 column steps in LDS, 1 register panels
+        int waves;         // wavefronts per workgroup: 2, 4, 8, 16 (method 0) or 8, 16 (method 1)
         int per_cu;        // workgroups per CU the grid is sized for
         int grid;          // min(count, CUs * per_cu)
         std::vector<int> order;  // the bucket's systems, most expensive first
@@ -536,7 +721,8 @@ int ragged_upload_bucket(gputils_gels_plan *p, const gputils_gels_plan::Bucket &
     return GPUTILS_OK;
 }
 
-void ragged_set_shape(const gputils_gels_plan *p, gputils_gels_plan::Bucket &bk, int waves, int per_cu) {
+void ragged_set_shape(const gputils_gels_plan *p, gputils_gels_plan::Bucket &bk, int method, int waves, int per_cu) {
+    bk.method = method;
     bk.waves = waves;
     bk.per_cu = per_cu;
     bk.grid = (int)std::min<int64_t>(bk.count, (int64_t)p->cus * per_cu);
@@ -674,8 +860,9 @@ int gputils_gels_ragged_plan_create(gputils_gels_plan_t *plan, int64_t count, co
         b.count = (int)mem.size();
         b.wbase = begin + (int)p->buckets.size();
         b.order = std::move(mem);
-        // untuned shape: eight wavefronts, as many workgroups per CU as fit
-        ragged_set_shape(p, b, 8, ragged_max_per_cu(b.maxe, 8));
+        // untuned shape: column steps, eight wavefronts, as many workgroups
+        // per CU as fit
+        ragged_set_shape(p, b, 0, 8, ragged_max_per_cu(b.maxe, 8));
         begin += b.count;
         p->buckets.push_back(std::move(b));
     }
@@ -710,21 +897,23 @@ int gputils_gels_ragged_plan_describe(gputils_gels_plan_t plan, int32_t *desc, i
     if (nbuckets) *nbuckets = (int)plan->buckets.size();
     for (int i = 0; i < capacity && i < (int)plan->buckets.size(); ++i) {
         const auto &bk = plan->buckets[(size_t)i];
-        const int32_t v[5] = {bk.maxe, bk.count, bk.waves, bk.per_cu, bk.grid};
-        for (int q = 0; q < 5; ++q) desc[5 * i + q] = v[q];
+        const int32_t v[6] = {bk.maxe, bk.count, bk.method, bk.waves, bk.per_cu, bk.grid};
+        for (int q = 0; q < 6; ++q) desc[6 * i + q] = v[q];
     }
     return GPUTILS_OK;
 }
 
-int gputils_gels_ragged_plan_configure(gputils_gels_plan_t plan, int bucket, int waves, int per_cu) {
+int gputils_gels_ragged_plan_configure(gputils_gels_plan_t plan, int bucket, int method, int waves, int per_cu) {
     GPUTILS_ARG(plan != nullptr, 1, "plan is NULL");
     GPUTILS_ARG(bucket >= 0 && bucket < (int)plan->buckets.size(), 2, "bucket out of range");
-    GPUTILS_ARG(waves == 2 || waves == 4 || waves == 8 || waves == 16, 3, "waves must be 2, 4, 8 or 16");
-    GPUTILS_ARG(per_cu >= 1 && per_cu <= 64, 4, "per_cu must be in [1, 64]");
+    GPUTILS_ARG(method == 0 || method == 1, 3, "method must be 0 or 1");
+    GPUTILS_ARG(method == 0 ? (waves == 2 || waves == 4 || waves == 8 || waves == 16) : (waves == 8 || waves == 16), 4,
+                "waves must be 2, 4, 8 or 16 (method 0) or 8 or 16 (method 1)");
+    GPUTILS_ARG(per_cu >= 1 && per_cu <= 64, 5, "per_cu must be in [1, 64]");
     // the device tables change: no execution of the plan may be pending
     GPUTILS_HIP(hipDeviceSynchronize());
     auto &bk = plan->buckets[(size_t)bucket];
-    ragged_set_shape(plan, bk, waves, per_cu);
+    ragged_set_shape(plan, bk, method, waves, per_cu);
     return ragged_upload_bucket(plan, bk);
 }
 
@@ -748,15 +937,18 @@ int gputils_gels_ragged_plan_tune(gputils_gels_plan_t plan, const double *A, con
     constexpr int kRepeats = 3;
     for (size_t bi = 0; rc == GPUTILS_OK && bi < plan->buckets.size(); ++bi) {
         auto &bk = plan->buckets[bi];
-        int best_w = bk.waves, best_pc = bk.per_cu;
+        int best_m = bk.method, best_w = bk.waves, best_pc = bk.per_cu;
         float best_ms = -1.0f;
-        std::vector<std::pair<int, int>> seen;  // (waves, grid) already timed
-        for (int waves : {2, 4, 8, 16}) {
+        std::vector<std::pair<int, int>> seen;  // (method * 64 + waves, grid) already timed
+        static const int kShapes[][2] = {{0, 2}, {0, 4}, {0, 8}, {0, 16}, {1, 8}, {1, 16}};
+        for (const auto &sh : kShapes) {
+            const int method = sh[0], waves = sh[1];
             const int cap = ragged_max_per_cu(bk.maxe, waves);
             for (int pc = 1; pc <= cap && rc == GPUTILS_OK; ++pc) {
-                ragged_set_shape(plan, bk, waves, pc);
-                if (std::find(seen.begin(), seen.end(), std::make_pair(waves, bk.grid)) != seen.end()) continue;
-                seen.push_back({waves, bk.grid});
+                ragged_set_shape(plan, bk, method, waves, pc);
+                const auto key = std::make_pair(method * 64 + waves, bk.grid);
+                if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
+                seen.push_back(key);
                 rc = ragged_upload_bucket(plan, bk);
                 float t_min = -1.0f;
                 for (int r = 0; rc == GPUTILS_OK && r <= kRepeats; ++r) {  // run 0 warms up
@@ -766,8 +958,8 @@ int gputils_gels_ragged_plan_tune(gputils_gels_plan_t plan, const double *A, con
                         rc = hip_fail(e, "tune");
                         break;
                     }
-                    rc = qr::launch_ragged(bk.maxe, bk.waves, bk.grid, st, plan->d_items, plan->d_wstart + bk.wbase, A,
-                                           scratch, refine_steps);
+                    rc = qr::launch_ragged(bk.maxe, bk.method, bk.waves, bk.grid, st, plan->d_items,
+                                           plan->d_wstart + bk.wbase, A, scratch, refine_steps);
                     if (rc != GPUTILS_OK) break;
                     e = hipEventRecord(ev1, st);
                     if (e == hipSuccess) e = hipEventSynchronize(ev1);
@@ -781,12 +973,13 @@ int gputils_gels_ragged_plan_tune(gputils_gels_plan_t plan, const double *A, con
                 }
                 if (rc == GPUTILS_OK && (best_ms < 0.0f || t_min < best_ms)) {
                     best_ms = t_min;
+                    best_m = method;
                     best_w = waves;
                     best_pc = pc;
                 }
             }
         }
-        ragged_set_shape(plan, bk, best_w, best_pc);
+        ragged_set_shape(plan, bk, best_m, best_w, best_pc);
         if (rc == GPUTILS_OK) rc = ragged_upload_bucket(plan, bk);
     }
     if (ev0) (void)hipEventDestroy(ev0);
@@ -807,8 +1000,8 @@ int gputils_dgels_ragged(gputils_gels_plan_t plan, const double *A, double *B, i
     GPUTILS_ARG(B != nullptr, 3, "B is NULL");
     hipStream_t st = as_stream(stream);
     for (const auto &bk : plan->buckets) {
-        int rc = qr::launch_ragged(bk.maxe, bk.waves, bk.grid, st, plan->d_items, plan->d_wstart + bk.wbase, A, B,
-                                   refine_steps);
+        int rc = qr::launch_ragged(bk.maxe, bk.method, bk.waves, bk.grid, st, plan->d_items,
+                                   plan->d_wstart + bk.wbase, A, B, refine_steps);
         if (rc) return rc;
     }
     return GPUTILS_OK;

@@ -209,28 +209,32 @@ int gputils_dgels_ragged(gputils_gels_plan_t plan, const double *A, double *B, i
                          void *stream);
 /* Launch shape per bucket.  A plan has one bucket per LDS size class in use
  * (FP32 m*n <= 2048, 4096, 8192, 16384, 32768; largest first), one launch
- * each.  Its shape: wavefronts per workgroup (2, 4, 8 or 16) and workgroups per
- * CU (the grid is min(systems in the bucket, CUs * per_cu)); the systems
- * are assigned to the grid's workgroups longest-processing-time first.  A
- * new plan uses 8 wavefronts and as many workgroups per CU as fit (LDS,
- * 32 wavefronts per CU).
- * The shape changes the speed only: every shape gives bit-identical
- * results.
+ * each.  Its shape: the factorisation method (0: A staged in LDS and
+ * factorised column by column, one barrier per column; 1: panels of
+ * 64 / waves columns held in registers, one barrier per panel), the
+ * wavefronts per workgroup (2, 4, 8 or 16 for method 0; 8 or 16 for
+ * method 1) and the workgroups per CU (the grid is min(systems in the
+ * bucket, CUs * per_cu)); the systems are assigned to the grid's
+ * workgroups longest-processing-time first.  A new plan uses method 0, 8
+ * wavefronts and as many workgroups per CU as fit (LDS, 32 wavefronts per
+ * CU).  For one method every shape gives bit-identical results; the two
+ * methods round the FP32 factorisation differently (the refined solutions
+ * agree to the refinement's accuracy).
  *
- * describe: writes {maxe, systems, waves, per_cu, grid} for the first
- * min(capacity, buckets) buckets to desc (5 int32 each) and the bucket count
- * to *nbuckets (if not NULL).
+ * describe: writes {maxe, systems, method, waves, per_cu, grid} for the
+ * first min(capacity, buckets) buckets to desc (6 int32 each) and the
+ * bucket count to *nbuckets (if not NULL).
  * configure: sets one bucket's shape.
- * tune: the autotuner.  For every bucket it times each shape (waves 2/4/8/16
- * times every per_cu the CU can hold) on the caller's A and a device copy
- * of B (B itself is not written), with refine_steps refinements, on
- * stream, and keeps the fastest.  It allocates the copy of B and so is a
- * plan-time call like plan_create.
+ * tune: the autotuner.  For every bucket it times each shape (both
+ * methods, their wavefront counts, every per_cu the CU can hold) on the
+ * caller's A and a device copy of B (B itself is not written), with
+ * refine_steps refinements, on stream, and keeps the fastest.  It
+ * allocates the copy of B and so is a plan-time call like plan_create.
  * configure and tune synchronise the device first and rewrite the plan's
  * device tables: no execution of the plan may be pending or run
  * concurrently. */
 int gputils_gels_ragged_plan_describe(gputils_gels_plan_t plan, int32_t *desc, int capacity, int *nbuckets);
-int gputils_gels_ragged_plan_configure(gputils_gels_plan_t plan, int bucket, int waves, int per_cu);
+int gputils_gels_ragged_plan_configure(gputils_gels_plan_t plan, int bucket, int method, int waves, int per_cu);
 int gputils_gels_ragged_plan_tune(gputils_gels_plan_t plan, const double *A, const double *B, int refine_steps,
                                   void *stream);
 

@@ -193,44 +193,80 @@ def test_ragged_plan_buckets_and_default_shape():
     desc = plan.describe()
     assert [(d["maxe"], d["systems"]) for d in desc] == _bucket_sizes(ms, ns)
     for d in desc:
-        assert d["waves"] == 8 and d["per_cu"] >= 1 and 1 <= d["grid"] <= d["systems"]
+        assert d["method"] == 0 and d["waves"] == 8 and d["per_cu"] >= 1 and 1 <= d["grid"] <= d["systems"]
 
 
-def test_ragged_every_launch_shape_gives_identical_results():
-    """The launch shape (wavefronts per workgroup, workgroups per CU and so
-    the assignment of systems to workgroups) changes no system's arithmetic:
-    every shape reproduces the default shape's solutions bit for bit, also
-    where a workgroup solves several systems in turn (per_cu = 1)."""
+_SHAPES = {0: (2, 4, 8, 16), 1: (8, 16)}
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_ragged_every_launch_shape_gives_identical_results(method):
+    """For one factorisation method the launch shape (wavefronts per
+    workgroup, workgroups per CU and so the assignment of systems to
+    workgroups) changes no system's arithmetic: every shape reproduces the
+    method's first shape bit for bit, also where a workgroup solves several
+    systems in turn (per_cu = 1)."""
     from gputils_amd import workloads
     ms, ns, offA, offB, Af, Bf = workloads.ls_ragged()
     plan = api.RaggedLeastSquares(ms, ns, offA, offB)
     tA = dev(Af)
-    ref = _ragged_solve(plan, tA, Bf, 1)
     nb = len(plan.describe())
-    for waves in (2, 4, 8, 16):
-        for per_cu in (1, 3):
+    ref = None
+    for waves in _SHAPES[method]:
+        for per_cu in (1, 2, 3):
             for b in range(nb):
-                plan.configure(b, waves, per_cu)
+                plan.configure(b, method, waves, per_cu)
             for d in plan.describe():
-                assert d["waves"] == waves and d["per_cu"] == per_cu and 1 <= d["grid"] <= d["systems"]
+                assert (d["method"], d["waves"], d["per_cu"]) == (method, waves, per_cu)
+                assert 1 <= d["grid"] <= d["systems"]
             got = _ragged_solve(plan, tA, Bf, 1)
+            if ref is None:
+                ref = got
             assert np.array_equal(got, ref), (waves, per_cu)
+
+
+def test_ragged_register_panel_method_matches_oracle(oracle):
+    """Method 1 (register panels) to the same bar as the default method:
+    one and two refinement steps against the FP64 oracle dgels."""
+    from gputils_amd import workloads
+    ms, ns, offA, offB, Af, Bf = workloads.ls_ragged()
+    plan = api.RaggedLeastSquares(ms, ns, offA, offB)
+    for b in range(len(plan.describe())):
+        plan.configure(b, 1, 16, 1)
+    tA = dev(Af)
+    rng = np.random.default_rng(0)
+    sample = rng.choice(len(ms), 200, replace=False)
+    for steps, bound in ((1, 1e-9), (2, 1e-10)):
+        out = _ragged_solve(plan, tA, Bf, steps)
+        for s in sample:
+            m, n = int(ms[s]), int(ns[s])
+            A = Af[offA[s]:offA[s] + m * n].reshape(n, m).T
+            b = Bf[offB[s]:offB[s] + m]
+            ref = oracle.gels(A, b).ravel()
+            x = out[offB[s]:offB[s] + n]
+            assert np.array_equal(out[offB[s] + n:offB[s] + m], b[n:]), s
+            err = np.linalg.norm(x - ref) / np.linalg.norm(ref)
+            assert err <= bound * max(np.linalg.cond(A), 1.0), (steps, s, m, n, err)
 
 
 def test_ragged_autotune_keeps_b_and_results():
     """The autotuner times the shapes on a copy of B (B is not written) and
-    leaves a valid shape per bucket that solves bit-identically."""
+    leaves a valid shape per bucket; the tuned plan solves bit-identically
+    to a plan given the same shapes by hand."""
     from gputils_amd import workloads
     ms, ns, offA, offB, Af, Bf = workloads.ls_ragged()
     plan = api.RaggedLeastSquares(ms, ns, offA, offB)
     tA, tB = dev(Af), dev(Bf)
-    ref = _ragged_solve(plan, tA, Bf, 1)
     desc = plan.tune(tA, tB, refine_steps=1)
     assert np.array_equal(tB.cpu().numpy(), Bf)
     assert [(d["maxe"], d["systems"]) for d in desc] == _bucket_sizes(ms, ns)
     for d in desc:
-        assert d["waves"] in (2, 4, 8, 16) and d["per_cu"] >= 1 and 1 <= d["grid"] <= d["systems"]
-    assert np.array_equal(_ragged_solve(plan, tA, Bf, 1), ref)
+        assert d["waves"] in _SHAPES[d["method"]] and d["per_cu"] >= 1 and 1 <= d["grid"] <= d["systems"]
+    tuned = _ragged_solve(plan, tA, Bf, 1)
+    other = api.RaggedLeastSquares(ms, ns, offA, offB)
+    for b, d in enumerate(desc):
+        other.configure(b, d["method"], d["waves"], d["per_cu"])
+    assert np.array_equal(_ragged_solve(other, tA, Bf, 1), tuned)
 
 
 def test_ragged_plan_configure_rejects_bad_shapes():
@@ -238,11 +274,15 @@ def test_ragged_plan_configure_rejects_bad_shapes():
     ms, ns, offA, offB, Af, Bf = workloads.ls_ragged(count=64)
     plan = api.RaggedLeastSquares(ms, ns, offA, offB)
     with pytest.raises(api.GputilsError):
-        plan.configure(0, 3, 1)       # waves must be 2, 4, 8 or 16
+        plan.configure(0, 2, 8, 1)    # method must be 0 or 1
     with pytest.raises(api.GputilsError):
-        plan.configure(0, 4, 0)       # per_cu >= 1
+        plan.configure(0, 1, 4, 1)    # method 1 takes 8 or 16 wavefronts
     with pytest.raises(api.GputilsError):
-        plan.configure(len(plan.describe()), 4, 1)  # no such bucket
+        plan.configure(0, 0, 3, 1)    # method 0 takes 2, 4, 8 or 16
+    with pytest.raises(api.GputilsError):
+        plan.configure(0, 0, 4, 0)    # per_cu >= 1
+    with pytest.raises(api.GputilsError):
+        plan.configure(len(plan.describe()), 0, 4, 1)  # no such bucket
 
 
 def test_ragged_plan_rejects_bad_sizes():
