@@ -546,14 +546,27 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void po
     for (int r0 = 0; r0 < nrhs; r0 += kRhs) {
         const int nr = (nrhs - r0 < kRhs) ? (nrhs - r0) : kRhs;
         const double *bb = B + b * strideB + (int64_t)r0 * ldb;
-        // b (zero-padded) into RD: the first right-hand side; x = 0
+        // b (zero-padded) into RD: the first right-hand side; x = 0.  Four
+        // loads per lane in flight at a time (one at a time, the eight
+        // round trips cost as much as a solve)
+        static_assert(NP * kRhs == 64 * NB, "b staging: NB entries per lane");
 #pragma unroll 1
-        for (int e = lane; e < NP * kRhs; e += 64) {
-            const int row = e >> 2, q = e & 3;
-            const bool in = row < n && q < nr;
-            const double v = bb[in ? row + (int64_t)q * ldb : 0];
-            RD[e] = in ? v : 0.0;
-            XD[e] = 0.0;
+        for (int t0 = 0; t0 < NB; t0 += 4) {
+            double v[4];
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int e = lane + 64 * (t0 + g), row = e >> 2, q = e & 3;
+                const bool in = t0 + g < NB && row < n && q < nr;
+                v[g] = bb[in ? row + (int64_t)q * ldb : 0];
+            }
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int e = lane + 64 * (t0 + g), row = e >> 2, q = e & 3;
+                if (t0 + g < NB) {
+                    RD[e] = (row < n && q < nr) ? v[g] : 0.0;
+                    XD[e] = 0.0;
+                }
+            }
         }
         double bn[kRhs];
         col_absmax<NP>(RD, lane, bn);
