@@ -21,6 +21,38 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
+// The scalars of a reflector from alpha and the squared norm sq > 0 of the
+// rest of the column: beta = -sign(alpha) sqrt(alpha^2 + sq),
+// tau = (beta - alpha) / beta, scale = 1 / (alpha - beta).  When the
+// factors are stored in FP32 but the scalars are formed in FP64, float
+// accuracy is all that survives: the square root is the hardware reciprocal
+// square root with one Newton step, and one reciprocal of
+// d beta (d = alpha - beta, |d| = |alpha| + norm, so no cancellation; the
+// reciprocal with one Newton step) gives tau = -d^2 / (d beta) and
+// 1 / d = beta / (d beta).  Otherwise the square root and the divisions
+// are the exact ones.
+template <typename T, typename ACC>
+__device__ __forceinline__ void reflector_scalars(ACC alpha, ACC sq, ACC &beta, ACC &tau, ACC &scale) {
+    if constexpr (sizeof(T) < sizeof(ACC)) {
+        const ACC s2 = alpha * alpha + sq;
+        ACC r = __builtin_amdgcn_rsq(s2);
+        r = r * fma(ACC(-0.5) * s2, r * r, ACC(1.5));
+        const ACC norm = s2 * r;
+        beta = (alpha >= ACC(0)) ? -norm : norm;
+        const ACC d = alpha - beta;
+        const ACC db = d * beta;
+        ACC inv = __builtin_amdgcn_rcp(db);
+        inv = inv * fma(-db, inv, ACC(2));
+        tau = -d * d * inv;
+        scale = beta * inv;
+    } else {
+        const ACC norm = sqrt(alpha * alpha + sq);
+        beta = (alpha >= ACC(0)) ? -norm : norm;
+        tau = (beta - alpha) / beta;
+        scale = ACC(1) / (alpha - beta);
+    }
+}
+
 // Reflector of column j (one wavefront).  Scales v in place, writes beta to
 // A[j][j] and returns tau (identical in every lane).
 template <typename T, typename ACC>
@@ -35,10 +67,9 @@ __device__ __forceinline__ T make_reflector(T *A, int64_t lda, int m, int j) {
     s = wave_sum(s);
     const ACC alpha = (ACC)col[j];
     if (s == ACC(0)) return T(0);  // H = I (dlarfg with x = 0)
-    const ACC norm = sqrt(alpha * alpha + s);
-    const ACC beta = (alpha >= ACC(0)) ? -norm : norm;
-    const ACC tau = (beta - alpha) / beta;
-    const T scale = (T)(ACC(1) / (alpha - beta));
+    ACC beta, tau, sc;
+    reflector_scalars<T, ACC>(alpha, s, beta, tau, sc);
+    const T scale = (T)sc;
     for (int i = j + 1 + lane; i < m; i += kWave) col[i] *= scale;
     wave_sync();
     if (lane == 0) col[j] = (T)beta;
@@ -84,10 +115,10 @@ __device__ __forceinline__ T update_and_reflect(T *A, int64_t lda, int m, int j,
     const ACC alpha = (ACC)readlane(xr[0], 0);
     T t1 = T(0);
     if (sq != ACC(0)) {  // else H = I (dlarfg with x = 0): the column stays as it is
-        const ACC norm = sqrt(alpha * alpha + sq);
-        const ACC beta = (alpha >= ACC(0)) ? -norm : norm;
-        t1 = (T)((beta - alpha) / beta);
-        const T scale = (T)(ACC(1) / (alpha - beta));
+        ACC beta, tau, sc;
+        reflector_scalars<T, ACC>(alpha, sq, beta, tau, sc);
+        t1 = (T)tau;
+        const T scale = (T)sc;
 #pragma unroll
         for (int u = 0; u < R; ++u) xr[u] = (u == 0 && lane == 0) ? (T)beta : xr[u] * scale;
     }

@@ -251,10 +251,10 @@ __device__ __forceinline__ void hqr_panels(const double *__restrict__ a, const d
         sq = wave_sum(sq);
         float t1 = 0.0f;
         if (sq != 0.0) {
-            const double norm = sqrt(alpha * alpha + sq);
-            const double beta = (alpha >= 0.0) ? -norm : norm;
-            t1 = (float)((beta - alpha) / beta);
-            const float scale = (float)(1.0 / (alpha - beta));
+            double beta, tau_d, sc;
+            reflector_scalars<float, double>(alpha, sq, beta, tau_d, sc);
+            t1 = (float)tau_d;
+            const float scale = (float)sc;
 #pragma unroll
             for (int u = 0; u < R; ++u) {
                 const int i = lane + 64 * u;
