@@ -21,7 +21,7 @@ __all__ = [
     "GputilsError", "from_matrices", "to_matrices", "stream_handle",
     "potrf_batched", "potrs_batched", "posv_batched", "posv_mixed_batched",
     "gemm_batched", "dot", "nrm2", "asum", "amax", "amin", "scal", "axpy",
-    "transpose_batched",
+    "transpose_batched", "givens_rotate_cols", "givens_rotate_rows", "givens_annihilate",
 ]
 
 
@@ -203,6 +203,53 @@ def transpose_batched(A: torch.Tensor) -> torch.Tensor:
     check(getattr(lib(), f"gputils_{_pfx(A)}transpose_batched")(m, n, _ptr(A), m, m * n, _ptr(B), n, m * n, k,
                                                                  stream_handle()), "transpose_batched")
     return B
+
+
+# ------------------------------------------------------------- Givens --
+def _matrix_dims(A: torch.Tensor):
+    """(m, n) of one matrix stored as (n, m) or (1, n, m) (column-major)."""
+    _check_dev(A, "A")
+    if A.dim() == 3:
+        if A.shape[0] != 1:
+            raise GputilsError("Givens rotations take one matrix (numMats = 1)")
+        return A.shape[2], A.shape[1]
+    if A.dim() != 2:
+        raise GputilsError("A must be (n, m) or (1, n, m)")
+    return A.shape[1], A.shape[0]
+
+
+def givens_rotate_cols(A: torch.Tensor, i: int, j: int, c: float, minus_s: float) -> torch.Tensor:
+    """DTensor::applyRightGivensRotation: columns i, j of the m x n matrix
+    (storage (n, m)) become c col_i + s col_j and c col_j - s col_i, s = minus_s."""
+    m, n = _matrix_dims(A)
+    if not (0 <= i < n and 0 <= j < n):
+        raise GputilsError("column index out of range")
+    es, base = A.element_size(), A.data_ptr()
+    fn = getattr(lib(), f"gputils_{_pfx(A)}rot")
+    check(fn(m, ctypes.c_void_p(base + es * m * i), 1, ctypes.c_void_p(base + es * m * j), 1, c, minus_s,
+             stream_handle()), "rot")
+    return A
+
+
+def givens_rotate_rows(A: torch.Tensor, i: int, j: int, c: float, minus_s: float) -> torch.Tensor:
+    """DTensor::applyLeftGivensRotation: rows i, j rotated likewise."""
+    m, n = _matrix_dims(A)
+    if not (0 <= i < m and 0 <= j < m):
+        raise GputilsError("row index out of range")
+    es, base = A.element_size(), A.data_ptr()
+    fn = getattr(lib(), f"gputils_{_pfx(A)}rot")
+    check(fn(n, ctypes.c_void_p(base + es * i), m, ctypes.c_void_p(base + es * j), m, c, minus_s,
+             stream_handle()), "rot")
+    return A
+
+
+def givens_annihilate(A: torch.Tensor, i: int, k: int, j: int) -> torch.Tensor:
+    """GivensAnnihilator::annihilate(i, k, j): rows i and k rotated so that
+    A(k, j) becomes zero (one launch; c and s stay on the device)."""
+    m, n = _matrix_dims(A)
+    fn = getattr(lib(), f"gputils_{_pfx(A)}givens_annihilate")
+    check(fn(m, n, _ptr(A), m, i, k, j, stream_handle()), "givens_annihilate")
+    return A
 
 
 # ------------------------------------------------------------ QR / LS --

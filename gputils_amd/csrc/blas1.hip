@@ -1,8 +1,9 @@
-// blas1.hip -- whole-tensor reductions, scal/axpy and batched transpose.
+// blas1.hip -- whole-tensor reductions, scal/axpy, batched transpose and
+// Givens rotations.
 //
 // Replace the cuBLAS level-1 calls behind DTensor::dotF/normF/sumAbs/
-// maxAbs/minAbs, *=, +=, -= and tr() (include/tensor.cuh:968-1072,
-// 1167-1276).  Reductions are two launches: a grid-stride pass writing one
+// maxAbs/minAbs, *=, +=, -=, tr() and applyLeft/RightGivensRotation, and
+// GivensAnnihilator (include/tensor.cuh:968-1104, 1167-1276, 2204-2312).  Reductions are two launches: a grid-stride pass writing one
 // partial per workgroup into the caller's workspace, then one workgroup
 // combining the partials.  All accumulation happens in double.
 #include "common.hpp"
@@ -136,6 +137,51 @@ template <typename T>
 __global__ __launch_bounds__(kThreads) void axpy_kernel(int64_t n, T alpha, const T *x, T *y) {
     for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
         y[i] += alpha * x[i];
+}
+
+// Plane rotation of two strided vectors (BLAS rot, behind
+// DTensor::applyRight/LeftGivensRotation, tensor.cuh:1074-1104):
+// x_k <- c x_k + s y_k, y_k <- c y_k - s x_k.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void rot_kernel(int64_t n, T *x, int64_t incx, T *y, int64_t incy, T c,
+                                                       T s) {
+    for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
+        const T xv = x[k * incx], yv = y[k * incy];
+        x[k * incx] = c * xv + s * yv;
+        y[k * incy] = c * yv - s * xv;
+    }
+}
+
+// 1 / hypot(a, b) with the squares formed after scaling by the larger
+// magnitude (no overflow or underflow): m = max(|a|, |b|), t = min / m,
+// 1 / (m sqrt(1 + t^2)); +inf for a = b = 0, like 1 / hypot(0, 0).
+template <typename T>
+__device__ inline T rhypot_scaled(T a, T b) {
+    const T aa = fabs(a), bb = fabs(b);
+    const T m = aa > bb ? aa : bb, lo = aa > bb ? bb : aa;
+    if (!(m > T(0))) return T(1) / m;
+    const T t = lo / m;
+    return T(1) / (m * sqrt(T(1) + t * t));
+}
+
+// GivensAnnihilator::annihilate (tensor.cuh:2250-2311) in one launch: every
+// thread reads A(i,j) and A(k,j), the workgroup synchronises, and rows i
+// and k are rotated by c = A(i,j) r, s = A(k,j) r (r = 1 / hypot), which
+// zeroes A(k,j).  The sines and cosines never leave the device.  One
+// workgroup: its barrier orders the two reads before the column-j update.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void givens_annihilate_kernel(int64_t ncols, T *A, int64_t lda, int64_t i,
+                                                                     int64_t k, int64_t j) {
+    const T xij = A[i + j * lda], xkj = A[k + j * lda];
+    const T r = rhypot_scaled(xij, xkj);
+    const T c = xij * r, s = xkj * r;
+    __syncthreads();
+    for (int64_t col = threadIdx.x; col < ncols; col += kThreads) {
+        T *p = A + col * lda;
+        const T xv = p[i], yv = p[k];
+        p[i] = c * xv + s * yv;
+        p[k] = c * yv - s * xv;
+    }
 }
 
 // B_b (n x m) = A_b^T (A_b m x n); 32x32 tiles through LDS, grid (tiles, batch).
@@ -289,5 +335,36 @@ int gputils_dtranspose_batched(int64_t m, int64_t n, const double *A, int64_t ld
     GPUTILS_ARG(B != nullptr, 6, "B is NULL");
     return transpose<double>(m, n, A, lda, strideA, B, ldb, strideB, batch, as_stream(stream));
 }
+
+#define GPUTILS_ROT_ENTRY(NAME, T)                                                                       \
+    int NAME(int64_t n, T *x, int64_t incx, T *y, int64_t incy, T c, T s, void *stream) {                 \
+        GPUTILS_ARG(n >= 0, 1, "n < 0");                                                                  \
+        GPUTILS_ARG(incx >= 1, 3, "incx < 1");                                                            \
+        GPUTILS_ARG(incy >= 1, 5, "incy < 1");                                                            \
+        if (n == 0) return GPUTILS_OK;                                                                    \
+        GPUTILS_ARG(x != nullptr, 2, "x is NULL");                                                        \
+        GPUTILS_ARG(y != nullptr, 4, "y is NULL");                                                        \
+        hipLaunchKernelGGL((rot_kernel<T>), dim3(grid_1d(n, kThreads * 4)), dim3(kThreads), 0,            \
+                           as_stream(stream), n, x, incx, y, incy, c, s);                                 \
+        return launch_check("rot_kernel");                                                                \
+    }
+GPUTILS_ROT_ENTRY(gputils_srot, float)
+GPUTILS_ROT_ENTRY(gputils_drot, double)
+
+#define GPUTILS_GIVENS_ENTRY(NAME, T)                                                                     \
+    int NAME(int64_t m, int64_t ncols, T *A, int64_t lda, int64_t i, int64_t k, int64_t j, void *stream) { \
+        GPUTILS_ARG(m >= 0, 1, "m < 0");                                                                  \
+        GPUTILS_ARG(ncols >= 0, 2, "ncols < 0");                                                          \
+        GPUTILS_ARG(lda >= (m > 1 ? m : 1), 4, "lda < max(1,m)");                                         \
+        GPUTILS_ARG(i >= 0 && i < m, 5, "row i out of range");                                            \
+        GPUTILS_ARG(k >= 0 && k < m, 6, "row k out of range");                                            \
+        GPUTILS_ARG(j >= 0 && j < ncols, 7, "column j out of range");                                     \
+        GPUTILS_ARG(A != nullptr, 3, "A is NULL");                                                        \
+        hipLaunchKernelGGL((givens_annihilate_kernel<T>), dim3(1), dim3(kThreads), 0, as_stream(stream),  \
+                           ncols, A, lda, i, k, j);                                                       \
+        return launch_check("givens_annihilate_kernel");                                                  \
+    }
+GPUTILS_GIVENS_ENTRY(gputils_sgivens_annihilate, float)
+GPUTILS_GIVENS_ENTRY(gputils_dgivens_annihilate, double)
 
 }  // extern "C"

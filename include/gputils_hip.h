@@ -99,6 +99,24 @@ int gputils_stranspose_batched(int64_t m, int64_t n, const float *A, int64_t lda
 int gputils_dtranspose_batched(int64_t m, int64_t n, const double *A, int64_t lda, int64_t strideA,
                                double *B, int64_t ldb, int64_t strideB, int64_t batch, void *stream);
 
+/* Plane rotation of two strided vectors: x_k <- c x_k + s y_k,
+ * y_k <- c y_k - s x_k for k < n, incx, incy >= 1 (cublas?rot behind
+ * DTensor::applyRightGivensRotation, columns i and j with increment 1, and
+ * applyLeftGivensRotation, rows i and j with increment numRows;
+ * tensor.cuh:1074-1104, which pass s = minus_s). */
+int gputils_srot(int64_t n, float *x, int64_t incx, float *y, int64_t incy, float c, float s, void *stream);
+int gputils_drot(int64_t n, double *x, int64_t incx, double *y, int64_t incy, double c, double s, void *stream);
+/* GivensAnnihilator::annihilate(i, k, j) (tensor.cuh:2261-2311 and its
+ * rhypot kernel 2250-2258) in one launch: with r = 1 / hypot(A(i,j),
+ * A(k,j)), c = A(i,j) r and s = A(k,j) r, rows i and k of the m x ncols
+ * matrix A are rotated as by gputils_?rot, which zeroes A(k,j) and puts
+ * hypot(A(i,j), A(k,j)) in A(i,j).  c and s never leave the device.
+ * Indices are checked (i, k < m; j < ncols). */
+int gputils_sgivens_annihilate(int64_t m, int64_t ncols, float *A, int64_t lda, int64_t i, int64_t k, int64_t j,
+                               void *stream);
+int gputils_dgivens_annihilate(int64_t m, int64_t ncols, double *A, int64_t lda, int64_t i, int64_t k, int64_t j,
+                               void *stream);
+
 /* ---------------------------------------------------------------- GEMM --- */
 /* C_i <- alpha A_i B_i + beta C_i; A_i m x k, B_i k x n, C_i m x n
  * (cublas?gemmBatched / ?gemm, tensor.cuh:1286-1338).  C may alias B

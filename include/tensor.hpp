@@ -502,6 +502,17 @@ public:
     /** Save as text, or binary .bt (tensor.cuh:797-825). */
     void saveToFile(std::string pathToFile);
 
+    /**
+     * Columns i and j rotated: col_i <- c col_i + s col_j,
+     * col_j <- c col_j - s col_i with s = *minus_s (tensor.cuh:1074-1086;
+     * c and minus_s are host pointers).  Matrices only (numMats = 1);
+     * i, j < numCols.
+     */
+    void applyRightGivensRotation(size_t i, size_t j, const T *c, const T *minus_s);
+
+    /** Rows i and j rotated likewise (tensor.cuh:1088-1104); i, j < numRows. */
+    void applyLeftGivensRotation(size_t i, size_t j, const T *c, const T *minus_s);
+
     /* ------------- operators ------------- */
 
     /** Shallow alias (tensor.cuh:1219-1228). */
@@ -755,6 +766,31 @@ DTensor<T> DTensor<T>::parseFromFile(std::string path_to_file, StorageMode mode)
     const bool binary = path_to_file.size() >= 3 && path_to_file.substr(path_to_file.size() - 3) == ".bt";
     data_t<T> parsed = binary ? vectorFromBinaryFile<T>(path_to_file) : vectorFromTextFile<T>(path_to_file);
     return DTensor<T>(parsed.data, parsed.numRows, parsed.numCols, parsed.numMats, mode);
+}
+
+template <typename T>
+void DTensor<T>::applyRightGivensRotation(size_t i, size_t j, const T *c, const T *minus_s) {
+    gputils_detail::require_fp<T>();
+    if (m_numMats > 1) throw std::invalid_argument("[applyRightGivensRotation] tensors (nMat>1) not supported");
+    if (i >= m_numCols || j >= m_numCols) throw std::invalid_argument("[applyRightGivensRotation] invalid column index");
+    T *col_i = m_d_data + i * m_numRows;
+    T *col_j = m_d_data + j * m_numRows;
+    if constexpr (std::is_same_v<T, double>)
+        gpuErrChk(gputils_drot((int64_t)m_numRows, col_i, 1, col_j, 1, *c, *minus_s, stream()));
+    if constexpr (std::is_same_v<T, float>)
+        gpuErrChk(gputils_srot((int64_t)m_numRows, col_i, 1, col_j, 1, *c, *minus_s, stream()));
+}
+
+template <typename T>
+void DTensor<T>::applyLeftGivensRotation(size_t i, size_t j, const T *c, const T *minus_s) {
+    gputils_detail::require_fp<T>();
+    if (m_numMats > 1) throw std::invalid_argument("[applyLeftGivensRotation] tensors (nMat>1) not supported");
+    if (i >= m_numRows || j >= m_numRows) throw std::invalid_argument("[applyLeftGivensRotation] invalid row index");
+    const int64_t ld = (int64_t)m_numRows;
+    if constexpr (std::is_same_v<T, double>)
+        gpuErrChk(gputils_drot((int64_t)m_numCols, m_d_data + i, ld, m_d_data + j, ld, *c, *minus_s, stream()));
+    if constexpr (std::is_same_v<T, float>)
+        gpuErrChk(gputils_srot((int64_t)m_numCols, m_d_data + i, ld, m_d_data + j, ld, *c, *minus_s, stream()));
 }
 
 template <typename T>
@@ -1108,6 +1144,49 @@ public:
 
     /** b_i <- P_i b_i in place (tensor.cuh:2081-2085). */
     void project(DTensor<T> &b) { b.addAB(*m_projOp, b, 1, 0); }
+};
+
+/* ------------------------------------------------------------ Givens --- */
+
+/**
+ * Zeroes element (k, j) of a matrix by a left Givens rotation of rows i and
+ * k (tensor.cuh:2204-2312).  annihilate(i, k, j) is one kernel launch that
+ * forms c = A(i,j) / hypot and s = A(k,j) / hypot on the device and
+ * rotates the two rows (the reference launches a one-thread kernel for the
+ * scalars, then cublas?rot in device pointer mode).  Index checks: i, k
+ * < numRows and j < numCols (the reference checks i and k only).
+ */
+TEMPLATE_WITH_TYPE_T
+class GivensAnnihilator {
+private:
+    DTensor<T> *m_matrix = nullptr;
+
+public:
+    GivensAnnihilator() { gputils_detail::require_fp<T>(); }
+
+    GivensAnnihilator(DTensor<T> &a) {
+        gputils_detail::require_fp<T>();
+        setMatrix(a);
+    }
+
+    void setMatrix(DTensor<T> &a) {
+        if (a.numMats() > 1) throw std::invalid_argument("[GivensAnnihilator] tensors (numMats > 1) not supported");
+        m_matrix = &a;
+    }
+
+    void annihilate(size_t i, size_t k, size_t j) {
+        if (!m_matrix) throw std::logic_error("[GivensAnnihilator::annihilate] no matrix set");
+        const size_t nR = m_matrix->numRows(), nC = m_matrix->numCols();
+        if (i >= nR || k >= nR) throw std::invalid_argument("[GivensAnnihilator::annihilate] invalid row index");
+        if (j >= nC) throw std::invalid_argument("[GivensAnnihilator::annihilate] invalid column index j");
+        const int64_t m = (int64_t)nR;
+        if constexpr (std::is_same_v<T, double>)
+            gpuErrChk(gputils_dgivens_annihilate(m, (int64_t)nC, m_matrix->raw(), m, (int64_t)i, (int64_t)k,
+                                                 (int64_t)j, m_matrix->stream()));
+        if constexpr (std::is_same_v<T, float>)
+            gpuErrChk(gputils_sgivens_annihilate(m, (int64_t)nC, m_matrix->raw(), m, (int64_t)i, (int64_t)k,
+                                                 (int64_t)j, m_matrix->stream()));
+    }
 };
 
 /* ------------------------------------------------------------ graphs --- */

@@ -21,6 +21,10 @@
  *   orc_dgesvd       golden sigma/U/V of the reference suite + scipy dgesvd
  *   orc_rank         golden ranks 3,2,1 of the reference suite
  *   orc_nullspace    A N = 0, N^T N = I, P = I - U_r U_r^T (closed form)
+ *   orc_drot         scipy BLAS drot, and the rotated values the reference
+ *                    suite expects of applyLeft/RightGivensRotation
+ *   orc_dgivens_annihilate  golden entries of the reference suite, A(k,j) = 0,
+ *                    A(i,j) = hypot, and the two rows' norms preserved
  */
 #include "oracle.h"
 
@@ -270,6 +274,26 @@ int orc_dgesvd(int64_t m, int64_t n, double *A, int64_t lda, double *S,
     free(ord);
     free(V);
     return sweeps;
+}
+
+/* ------------------------------------------------------------- Givens --- */
+/* tensor.cuh:1074-1104: cublas?rot(n, x, incx, y, incy, c, s), BLAS drot. */
+void orc_drot(int64_t n, double *x, int64_t incx, double *y, int64_t incy, double c, double s) {
+    for (int64_t k = 0; k < n; ++k) {
+        const double xv = x[k * incx], yv = y[k * incy];
+        x[k * incx] = c * xv + s * yv;
+        y[k * incy] = c * yv - s * xv;
+    }
+}
+
+/* tensor.cuh:2250-2258 (the rhypot kernel) and 2261-2311 (rot of rows i, k
+ * with the device-side c and s). */
+void orc_dgivens_annihilate(int64_t m, int64_t ncols, double *A, int64_t lda, int64_t i, int64_t k,
+                            int64_t j) {
+    (void)m;
+    const double xij = A[IDX(i, j, lda)], xkj = A[IDX(k, j, lda)];
+    const double r = 1.0 / hypot(xij, xkj);
+    orc_drot(ncols, A + i, lda, A + k, lda, xij * r, xkj * r);
 }
 
 /* tensor.cuh:1484-1491 (kernel) and 1600-1609: count sigma > eps. */

@@ -74,6 +74,18 @@ int orc_rank(int64_t n, const double *S, double eps);
 int orc_nullspace(int64_t m, int64_t n, const double *A, int64_t lda,
                   double eps, double *N, double *P);
 
+/* Plane rotation of two strided vectors, the BLAS drot definition behind
+ * DTensor::applyRight/LeftGivensRotation (tensor.cuh:1074-1104):
+ * x_k <- c x_k + s y_k, y_k <- c y_k - s x_k, k < n (incx, incy >= 1). */
+void orc_drot(int64_t n, double *x, int64_t incx, double *y, int64_t incy, double c, double s);
+
+/* GivensAnnihilator::annihilate (tensor.cuh:2261-2311, kernel 2250-2258):
+ * r = 1 / hypot(A(i,j), A(k,j)), c = A(i,j) r, s = A(k,j) r, then rows i
+ * and k of the m x ncols matrix A rotated by (c, s) as in orc_drot, which
+ * zeroes A(k,j). */
+void orc_dgivens_annihilate(int64_t m, int64_t ncols, double *A, int64_t lda, int64_t i, int64_t k,
+                            int64_t j);
+
 /* Batched host loops (OpenMP over matrices) used for the CPU baseline. */
 int orc_dposv_batched(int64_t n, int64_t nrhs, double *A, double *B,
                       int64_t batch, int threads);

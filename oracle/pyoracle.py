@@ -30,8 +30,11 @@ _i64p = ctypes.POINTER(ctypes.c_int64)
 
 
 def build(force: bool = False) -> str:
-    """Compile oracle.c with gcc (make) into liboracle.so next to this file."""
-    if force or not os.path.exists(_LIB_PATH):
+    """Compile oracle.c with gcc (make) into liboracle.so next to this file,
+    also when a source is newer than the library."""
+    stale = not os.path.exists(_LIB_PATH) or any(
+        os.path.getmtime(os.path.join(_HERE, f)) > os.path.getmtime(_LIB_PATH) for f in ("oracle.c", "oracle.h"))
+    if force or stale:
         subprocess.run(["make", "-s", "-B", "-C", _HERE], check=True)
     return _LIB_PATH
 
@@ -65,6 +68,8 @@ def lib():
         L.orc_nullspace_batched.argtypes = [_i64, _i64, _dp, ctypes.c_double, _dp, _dp, _i32p, _i64,
                                             ctypes.c_int]
         L.orc_nullspace_batched.restype = ctypes.c_int
+        L.orc_drot.argtypes = [_i64, _dp, _i64, _dp, _i64, ctypes.c_double, ctypes.c_double]
+        L.orc_dgivens_annihilate.argtypes = [_i64, _i64, _dp, _i64, _i64, _i64, _i64]
         _lib = L
     return _lib
 
@@ -149,6 +154,32 @@ def gesvd(A, compute_u=True):
     sweeps = lib().orc_dgesvd(m, n, _p(A), m, _p(S),
                               _p(U) if compute_u else None, m, _p(VT), n)
     return S[:n], U, VT, sweeps
+
+
+def rot_rows(A, i, k, c, s):
+    """Rows i and k of A (m x n) rotated as by DTensor::applyLeftGivensRotation:
+    row_i <- c row_i + s row_k, row_k <- c row_k - s row_i.  Returns a copy."""
+    F = np.asfortranarray(A, dtype=np.float64).copy(order="F")
+    m = F.shape[0]
+    ptr = F.ctypes.data
+    lib().orc_drot(F.shape[1], ctypes.cast(ptr + 8 * i, _dp), m, ctypes.cast(ptr + 8 * k, _dp), m, c, s)
+    return F
+
+
+def rot_cols(A, i, j, c, s):
+    """Columns i and j of A rotated as by DTensor::applyRightGivensRotation."""
+    F = np.asfortranarray(A, dtype=np.float64).copy(order="F")
+    m = F.shape[0]
+    ptr = F.ctypes.data
+    lib().orc_drot(m, ctypes.cast(ptr + 8 * m * i, _dp), 1, ctypes.cast(ptr + 8 * m * j, _dp), 1, c, s)
+    return F
+
+
+def givens_annihilate(A, i, k, j):
+    """GivensAnnihilator::annihilate(i, k, j) on a copy of A (m x n)."""
+    F = np.asfortranarray(A, dtype=np.float64).copy(order="F")
+    lib().orc_dgivens_annihilate(F.shape[0], F.shape[1], _p(F), F.shape[0], i, k, j)
+    return F
 
 
 def rank(S, eps=1e-6):

@@ -865,6 +865,108 @@ TEST_F(NullspaceTest, projectCapturedInGraph) {  // repeated project through Str
     }
 }
 
+/* ============================================================ Givens ==== */
+
+// X:580-603: columns 1 and 4 of the 10 x 6 iota matrix rotated
+template <typename T>
+static void tensorRightGivens(T epsilon) {
+    const size_t m = 10, n = 6;
+    std::vector<T> v(m * n);
+    std::iota(v.begin(), v.end(), T(1));
+    DTensor<T> a(v, m, n, 1);
+    const size_t ig = 1, jg = 4;
+    const T c = 0.1, minus_s = std::sqrt(1 - c * c);
+    a.applyRightGivensRotation(ig, jg, &c, &minus_s);
+    for (size_t i = 0; i < m; i++) {
+        EXPECT_NEAR(1 + i, a(i, 0), epsilon);
+        EXPECT_NEAR(21 + i, a(i, 2), epsilon);
+        EXPECT_NEAR(31 + i, a(i, 3), epsilon);
+        EXPECT_NEAR((11 + i) * c - (41 + i) * (-minus_s), a(i, ig), epsilon);
+        EXPECT_NEAR((11 + i) * (-minus_s) + (41 + i) * c, a(i, jg), epsilon);
+    }
+}
+
+TEST_F(TensorTest, tensorRightGivens) {
+    tensorRightGivens<float>(PRECISION_LOW);
+    tensorRightGivens<double>(PRECISION_HIGH);
+}
+
+// X:615-641: rows 1 and 9 rotated (the reference builds a double tensor in
+// both instantiations)
+template <typename T>
+static void tensorLeftGivens(T epsilon) {
+    const size_t m = 10, n = 6;
+    std::vector<double> v(m * n);
+    std::iota(v.begin(), v.end(), 1.0);
+    DTensor<double> a(v, m, n, 1);
+    const size_t ig = 1, jg = 9;
+    const double c = 0.1, minus_s = -std::sqrt(1 - c * c);
+    a.applyLeftGivensRotation(ig, jg, &c, &minus_s);
+    for (size_t j = 0; j < n; j++) {
+        EXPECT_NEAR(1 + 10 * j, a(0, j), epsilon);
+        for (size_t i = 2; i < m - 1; i++) EXPECT_NEAR(1 + i + 10 * j, a(i, j), epsilon);
+        EXPECT_NEAR((2 + 10 * j) * c + (10 + 10 * j) * minus_s, a(ig, j), epsilon);
+        EXPECT_NEAR((2 + 10 * j) * (-minus_s) + (10 + 10 * j) * c, a(jg, j), epsilon);
+    }
+}
+
+TEST_F(TensorTest, tensorLeftGivens) {
+    tensorLeftGivens<float>(1e-10);
+    tensorLeftGivens<double>(1e-14);
+}
+
+// X:1684-1701: every element below row 0 annihilated in turn
+template <typename T>
+static void givensAnnihilateElement(T epsilon) {
+    const size_t m = 10, n = 6;
+    std::vector<T> v(m * n);
+    std::iota(v.begin(), v.end(), T(1));
+    DTensor<T> a(v, m, n, 1);
+    GivensAnnihilator<T> ga(a);
+    for (size_t k = 1; k < m; k++) {
+        for (size_t j = 0; j < n; j++) {
+            ga.annihilate(0, k, j);
+            EXPECT_NEAR(0.0, a(k, j), epsilon);
+        }
+    }
+}
+
+TEST_F(GivensAnnihilatorTest, givensAnnihilateElement) {
+    givensAnnihilateElement<float>(PRECISION_LOW);
+    givensAnnihilateElement<double>(PRECISION_HIGH);
+}
+
+// X:1713-1734: golden entries after annihilating (1, 2) with row 0
+template <typename T>
+static void givensAnnihilateCorrectness(T epsilon) {
+    const size_t m = 10, n = 6;
+    std::vector<double> v(m * n);
+    std::iota(v.begin(), v.end(), 1.0);
+    DTensor<double> a(v, m, n);
+    GivensAnnihilator<double> ga(a);
+    ga.annihilate(0, 1, 2);
+    EXPECT_NEAR(0.0, a(1, 2), epsilon);
+    EXPECT_NEAR(2.137186834969645, a(0, 0), epsilon);
+    EXPECT_NEAR(44.552125559751822, a(0, 3), epsilon);
+    EXPECT_NEAR(-0.328797974610715, a(1, 3), epsilon);
+}
+
+TEST_F(GivensAnnihilatorTest, givensAnnihilateCorrectness) {
+    givensAnnihilateCorrectness<double>(1e-14);
+    givensAnnihilateCorrectness<float>(1e-12);
+}
+
+TEST_F(GivensAnnihilatorTest, invalidIndicesThrow) {  // no reference counterpart: j is checked too
+    std::vector<double> v(12, 1.0);
+    DTensor<double> a(v, 4, 3);
+    GivensAnnihilator<double> ga(a);
+    bool threw_row = false, threw_col = false;
+    try { ga.annihilate(0, 4, 0); } catch (const std::invalid_argument &) { threw_row = true; }
+    try { ga.annihilate(0, 1, 3); } catch (const std::invalid_argument &) { threw_col = true; }
+    EXPECT_TRUE(threw_row);
+    EXPECT_TRUE(threw_col);
+}
+
 /* ============================================================ main ====== */
 int main(int argc, char **argv) {
     const char *filter = argc > 1 ? argv[1] : nullptr;

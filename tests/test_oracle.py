@@ -230,3 +230,62 @@ def test_batched_loops_match_single(oracle):
     C = oracle.gemm_batched(Af, Bf, threads=2)
     for b in range(k):
         assert np.allclose(C[b].T, Af[b].T @ Bf[b].T, atol=1e-5)
+
+
+def _iota(m=10, n=6):
+    """The reference suite's test matrix: 1 .. m n, column-major m x n."""
+    return np.arange(1, m * n + 1, dtype=np.float64).reshape(n, m).T
+
+
+def test_rot_against_blas_and_reference_expectations(oracle):
+    """orc_drot is BLAS drot: scipy's drot on random vectors, and the
+    values the reference suite expects of applyRight/LeftGivensRotation
+    (X:580-603, X:615-641)."""
+    from scipy.linalg import blas
+    rng = np.random.default_rng(11)
+    X = rng.standard_normal((40, 2))
+    c, s = 0.28, -0.96
+    got = oracle.rot_cols(X, 0, 1, c, s)
+    ex, ey = blas.drot(X[:, 0].copy(), X[:, 1].copy(), c, s)
+    assert np.allclose(got[:, 0], ex, rtol=0, atol=1e-15)
+    assert np.allclose(got[:, 1], ey, rtol=0, atol=1e-15)
+    A = _iota()
+    c, ms = 0.1, np.sqrt(1 - 0.01)
+    R = oracle.rot_cols(A, 1, 4, c, ms)
+    i = np.arange(10)
+    assert np.array_equal(R[:, [0, 2, 3, 5]], A[:, [0, 2, 3, 5]])
+    assert np.allclose(R[:, 1], (11 + i) * c - (41 + i) * (-ms), rtol=0, atol=1e-12)
+    assert np.allclose(R[:, 4], (11 + i) * (-ms) + (41 + i) * c, rtol=0, atol=1e-12)
+    ms = -np.sqrt(1 - 0.01)
+    L = oracle.rot_rows(A, 1, 9, c, ms)
+    j = np.arange(6)
+    assert np.array_equal(L[[0] + list(range(2, 9))], A[[0] + list(range(2, 9))])
+    assert np.allclose(L[1], (2 + 10 * j) * c + (10 + 10 * j) * ms, rtol=0, atol=1e-14)
+    assert np.allclose(L[9], (2 + 10 * j) * (-ms) + (10 + 10 * j) * c, rtol=0, atol=1e-14)
+
+
+def test_givens_annihilate_golden_and_invariants(oracle):
+    """Golden entries of the reference suite (X:1713-1729), then what the
+    mathematics fixes: A(k,j) = 0, A(i,j) = hypot(A(i,j), A(k,j)), the
+    other rows untouched and each column's norm over rows i, k kept."""
+    A = _iota()
+    G = oracle.givens_annihilate(A, 0, 1, 2)
+    assert abs(G[1, 2]) <= 1e-14
+    assert abs(G[0, 0] - 2.137186834969645) <= 1e-14
+    assert abs(G[0, 3] - 44.552125559751822) <= 1e-14
+    assert abs(G[1, 3] - (-0.328797974610715)) <= 1e-14
+    rng = np.random.default_rng(12)
+    B = rng.standard_normal((9, 7))
+    for (i, k, j) in [(0, 5, 3), (4, 2, 0), (8, 1, 6)]:
+        H = oracle.givens_annihilate(B, i, k, j)
+        assert abs(H[k, j]) <= 1e-15 * np.hypot(B[i, j], B[k, j])
+        assert np.isclose(H[i, j], np.hypot(B[i, j], B[k, j]), rtol=1e-15, atol=0)
+        others = [r for r in range(9) if r not in (i, k)]
+        assert np.array_equal(H[others], B[others])
+        assert np.allclose(H[i] ** 2 + H[k] ** 2, B[i] ** 2 + B[k] ** 2, rtol=1e-14, atol=0)
+    # the reference loop (X:1684-1701): every element below row 0 zeroed in turn
+    C = _iota()
+    for k in range(1, 10):
+        for j in range(6):
+            C = oracle.givens_annihilate(C, 0, k, j)
+            assert abs(C[k, j]) <= 1e-10
