@@ -10,11 +10,8 @@
 // updates its row with A[i][j] -= A[i][k] A[j][k] / A[k][k].  The matrix is
 // padded with the identity to N = 16 or 32 rows so the unrolled code needs
 // no bounds checks.  The forward solve uses the row layout; the backward
-// solve (L^T x = y) reads the strictly lower triangle of L from an LDS copy
-// packed by rows (row K at K (K - 1) / 2), in which lane i reads L[K][i]
-// at consecutive addresses for every K; half the LDS of a square copy, so
-// twice the wavefronts fit on a CU.  Both solves multiply by the reciprocal
-// pivots kept from the factorisation instead of dividing.
+// solve (L^T x = y) reads L by columns from an LDS copy.  Both multiply by
+// the reciprocal pivots kept from the factorisation instead of dividing.
 #include "chol_small.hpp"
 #include "common.hpp"
 
@@ -31,16 +28,14 @@ __device__ __forceinline__ void static_for(F &&f) {
     }
 }
 
-// packed strictly lower triangle of L: element (row K, column i < K) at
-// tri(K) + i
-constexpr int tri(int k) { return k * (k - 1) / 2; }
+constexpr int kLdT = 33;  // LDS copy of L: element (row i, column j) at j * kLdT + i
 
 template <int N, bool WRITE_L>
 __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double *A, int64_t lda, int64_t strideA,
                                                         const double *B, int64_t ldb, int64_t strideB, double *X,
                                                         int64_t ldx, int64_t strideX, int *info, int64_t batch,
                                                         const int *mask) {
-    __shared__ __attribute__((aligned(16))) double LT[2][tri(N)];
+    __shared__ __attribute__((aligned(16))) double LT[2][N * kLdT];
     __shared__ __attribute__((aligned(16))) double CB[2][N];  // per-half broadcast buffer
     __shared__ double RI[2][N];                               // per-half 1 / L(k, k)
     const int lane = threadIdx.x, i = lane & 31;
@@ -100,11 +95,8 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double 
             if (J <= i && J < n) a_g[i + (int64_t)J * lda] = a[J];
         });
     }
-    // strictly lower triangle of L by rows for the backward solve (lanes
-    // past N and the identity padding write nothing)
-    static_for<0, N>([&](auto J) {
-        if (i < N && J < i) lt[tri(i) + J] = a[J];
-    });
+    // L by columns for the backward solve
+    static_for<0, N>([&](auto J) { lt[J * kLdT + i] = a[J]; });
 
     const double *bb = B + (active ? b : 0) * strideB;
     double *xx = X + (active ? b : 0) * strideX;
@@ -126,9 +118,7 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double 
             if (i < N) cb[i] = y;  // lanes past N own no row of the buffer
             asm volatile("" ::: "memory");
             const double xk = cb[K] * ri[K];
-            // L[K][i]; the index is clamped for the lanes that do not use it
-            const double lki = lt[tri(K) + ((i < K) ? i : 0)];
-            y = (i == K) ? xk : ((i < K) ? fma(-lki, xk, y) : y);
+            y = (i == K) ? xk : ((i < K) ? fma(-lt[i * kLdT + K], xk, y) : y);  // L[K][i]
         });
         if (ok && i < n) xx[i + (int64_t)r * ldx] = y;
     }
