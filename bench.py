@@ -54,6 +54,8 @@ def _args(argv=None):
     ap.add_argument("--ragged-shapes", default=None,
                     help="ls_ragged: launch shapes per bucket as METHOD:WAVES:PER_CU,... (largest bucket "
                          "first) applied instead of autotuning; the default-shape timing is then skipped")
+    ap.add_argument("--ragged-concurrent", type=int, choices=[0, 1], default=None,
+                    help="ls_ragged with --ragged-shapes: run the buckets concurrently (1) or in turn (0)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison runs")
     return ap.parse_args(argv)
 
@@ -426,10 +428,13 @@ def bench_ls_ragged(a):
             raise SystemExit(f"--ragged-shapes needs {len(default_shapes)} shapes, got {len(given)}")
         for b, (method, waves, per_cu) in enumerate(given):
             plan.configure(b, method, waves, per_cu)
-        ev_default, t_tune, shapes = None, None, plan.describe()
+        if a.ragged_concurrent is not None:
+            plan.concurrent = bool(a.ragged_concurrent)
+        ev_default, t_tune, shapes, default_concurrent = None, None, plan.describe(), None
         print(f"[bench] ls_ragged given shapes {shapes}", file=sys.stderr, flush=True)
     else:
         ev_default, _ = run(1)
+        default_concurrent = plan.concurrent
         torch.cuda.synchronize()
         t_tune = time.perf_counter()
         shapes = plan.tune(dA, dB0, refine_steps=1)
@@ -465,7 +470,8 @@ def bench_ls_ragged(a):
                            "default_shapes": [[d["maxe"], d["method"], d["waves"], d["per_cu"], d["grid"]]
                                               for d in default_shapes],
                            "shapes": [[d["maxe"], d["method"], d["waves"], d["per_cu"], d["grid"]] for d in shapes],
-                           "shape_fields": ["maxe", "method", "waves", "per_cu", "grid"]},
+                           "shape_fields": ["maxe", "method", "waves", "per_cu", "grid"],
+                           "default_concurrent_buckets": default_concurrent, "concurrent_buckets": plan.concurrent},
                  fp_rate_tflops=flops / (ev * 1e-3) / 1e12)
 
 

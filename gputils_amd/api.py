@@ -340,6 +340,17 @@ class RaggedLeastSquares:
         waves 2/4/8/16) or 1 (register panels, waves 8/16), workgroups per CU."""
         check(lib().gputils_gels_ragged_plan_configure(self._plan, bucket, method, waves, per_cu), "plan_configure")
 
+    @property
+    def concurrent(self) -> bool:
+        """Buckets run concurrently on the plan's streams (True) or in turn."""
+        v = ctypes.c_int(0)
+        check(lib().gputils_gels_ragged_plan_get_concurrency(self._plan, ctypes.byref(v)), "plan_concurrency")
+        return bool(v.value)
+
+    @concurrent.setter
+    def concurrent(self, on: bool):
+        check(lib().gputils_gels_ragged_plan_set_concurrency(self._plan, 1 if on else 0), "plan_concurrency")
+
     def tune(self, A_flat: torch.Tensor, B_flat: torch.Tensor, refine_steps: int = 1):
         """Autotune every bucket's launch shape on these inputs (B_flat is not
         written); returns describe()."""

@@ -669,6 +669,14 @@ struct gputils_gels_plan {
     std::vector<int64_t> offA, offB;
     std::vector<Bucket> buckets;
     int64_t extentB = 0;  // max over systems of offB + m
+    // bucket b > 0 runs on streams[b - 1]: forked from the caller's stream
+    // by `fork`, joined back by done[b - 1]
+    std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> done;
+    hipEvent_t fork = nullptr;
+    // buckets concurrently (default) or one after another on the caller's
+    // stream; the tuner keeps whichever runs the whole plan faster
+    bool concurrent = true;
 };
 
 namespace {
@@ -718,6 +726,28 @@ int ragged_upload_bucket(gputils_gels_plan *p, const gputils_gels_plan::Bucket &
                           hipMemcpyHostToDevice));
     GPUTILS_HIP(hipMemcpy(p->d_wstart + bk.wbase, wstart.data(), sizeof(int) * wstart.size(),
                           hipMemcpyHostToDevice));
+    return GPUTILS_OK;
+}
+
+// One execution of the plan on st.  Concurrent: buckets after the first
+// wait for the work before this call on st (fork), run on the plan's
+// streams, and st waits for all of them (join), so the call stays ordered
+// on st; otherwise every bucket runs on st in turn.
+int ragged_execute(gputils_gels_plan *plan, const double *A, double *B, int steps, hipStream_t st) {
+    const size_t nb = plan->buckets.size();
+    const bool conc = plan->concurrent && nb > 1;
+    if (conc) GPUTILS_HIP(hipEventRecord(plan->fork, st));
+    for (size_t b = 0; b < nb; ++b) {
+        const auto &bk = plan->buckets[b];
+        hipStream_t sb = (conc && b > 0) ? plan->streams[b - 1] : st;
+        if (conc && b > 0) GPUTILS_HIP(hipStreamWaitEvent(sb, plan->fork, 0));
+        int rc = qr::launch_ragged(bk.maxe, bk.method, bk.waves, bk.grid, sb, plan->d_items,
+                                   plan->d_wstart + bk.wbase, A, B, steps);
+        if (rc) return rc;
+        if (conc && b > 0) GPUTILS_HIP(hipEventRecord(plan->done[b - 1], sb));
+    }
+    if (conc)
+        for (size_t b = 1; b < nb; ++b) GPUTILS_HIP(hipStreamWaitEvent(st, plan->done[b - 1], 0));
     return GPUTILS_OK;
 }
 
@@ -866,15 +896,30 @@ int gputils_gels_ragged_plan_create(gputils_gels_plan_t *plan, int64_t count, co
         begin += b.count;
         p->buckets.push_back(std::move(b));
     }
+    // the buckets' launches run concurrently (each one's tail overlaps the
+    // others'): one stream and one event per bucket after the first
+    if (p->buckets.size() > 1) {
+        hipError_t e = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming);
+        for (size_t b = 1; e == hipSuccess && b < p->buckets.size(); ++b) {
+            hipStream_t sb = nullptr;
+            hipEvent_t eb = nullptr;
+            e = hipStreamCreateWithFlags(&sb, hipStreamNonBlocking);
+            if (e == hipSuccess) p->streams.push_back(sb);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&eb, hipEventDisableTiming);
+            if (e == hipSuccess) p->done.push_back(eb);
+        }
+        if (e != hipSuccess) {
+            gputils_gels_ragged_plan_destroy(p);
+            return hip_fail(e, "plan streams");
+        }
+    }
     if (count > 0) {
         hipError_t e = hipMalloc(&p->d_items, sizeof(qr::RaggedItem) * (size_t)count);
         if (e == hipSuccess) e = hipMalloc(&p->d_wstart, sizeof(int) * ((size_t)count + p->buckets.size()));
         int rc = e == hipSuccess ? GPUTILS_OK : hip_fail(e, "plan allocation");
         for (size_t i = 0; rc == GPUTILS_OK && i < p->buckets.size(); ++i) rc = ragged_upload_bucket(p, p->buckets[i]);
         if (rc != GPUTILS_OK) {
-            if (p->d_items) (void)hipFree(p->d_items);
-            if (p->d_wstart) (void)hipFree(p->d_wstart);
-            delete p;
+            gputils_gels_ragged_plan_destroy(p);
             return rc;
         }
     }
@@ -884,10 +929,17 @@ int gputils_gels_ragged_plan_create(gputils_gels_plan_t *plan, int64_t count, co
 
 int gputils_gels_ragged_plan_destroy(gputils_gels_plan_t plan) {
     if (plan == nullptr) return GPUTILS_OK;
-    if (plan->d_items) GPUTILS_HIP(hipFree(plan->d_items));
-    if (plan->d_wstart) GPUTILS_HIP(hipFree(plan->d_wstart));
+    hipError_t e = hipSuccess;
+    auto keep = [&e](hipError_t r) {
+        if (e == hipSuccess) e = r;
+    };
+    if (plan->d_items) keep(hipFree(plan->d_items));
+    if (plan->d_wstart) keep(hipFree(plan->d_wstart));
+    for (hipStream_t sb : plan->streams) keep(hipStreamDestroy(sb));
+    for (hipEvent_t eb : plan->done) keep(hipEventDestroy(eb));
+    if (plan->fork) keep(hipEventDestroy(plan->fork));
     delete plan;
-    return GPUTILS_OK;
+    return e == hipSuccess ? GPUTILS_OK : hip_fail(e, "plan destroy");
 }
 
 int gputils_gels_ragged_plan_describe(gputils_gels_plan_t plan, int32_t *desc, int capacity, int *nbuckets) {
@@ -982,6 +1034,33 @@ int gputils_gels_ragged_plan_tune(gputils_gels_plan_t plan, const double *A, con
         ragged_set_shape(plan, bk, best_m, best_w, best_pc);
         if (rc == GPUTILS_OK) rc = ragged_upload_bucket(plan, bk);
     }
+    // with the shapes fixed, the whole plan concurrently and in turn
+    if (rc == GPUTILS_OK && plan->buckets.size() > 1) {
+        float best[2] = {-1.0f, -1.0f};
+        for (int mode = 0; mode < 2 && rc == GPUTILS_OK; ++mode) {
+            plan->concurrent = mode == 1;
+            for (int r = 0; r <= kRepeats && rc == GPUTILS_OK; ++r) {
+                e = hipMemcpyAsync(scratch, B, sizeof(double) * (size_t)plan->extentB, hipMemcpyDeviceToDevice, st);
+                if (e == hipSuccess) e = hipEventRecord(ev0, st);
+                if (e != hipSuccess) {
+                    rc = hip_fail(e, "tune");
+                    break;
+                }
+                rc = ragged_execute(plan, A, scratch, refine_steps, st);
+                if (rc != GPUTILS_OK) break;
+                e = hipEventRecord(ev1, st);
+                if (e == hipSuccess) e = hipEventSynchronize(ev1);
+                float ms = 0.0f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev0, ev1);
+                if (e != hipSuccess) {
+                    rc = hip_fail(e, "tune timing");
+                    break;
+                }
+                if (r > 0 && (best[mode] < 0.0f || ms < best[mode])) best[mode] = ms;
+            }
+        }
+        plan->concurrent = rc == GPUTILS_OK && best[1] < best[0];
+    }
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (scratch) {
@@ -998,12 +1077,20 @@ int gputils_dgels_ragged(gputils_gels_plan_t plan, const double *A, double *B, i
     if (plan->count == 0) return GPUTILS_OK;
     GPUTILS_ARG(A != nullptr, 2, "A is NULL");
     GPUTILS_ARG(B != nullptr, 3, "B is NULL");
-    hipStream_t st = as_stream(stream);
-    for (const auto &bk : plan->buckets) {
-        int rc = qr::launch_ragged(bk.maxe, bk.method, bk.waves, bk.grid, st, plan->d_items,
-                                   plan->d_wstart + bk.wbase, A, B, refine_steps);
-        if (rc) return rc;
-    }
+    return ragged_execute(plan, A, B, refine_steps, as_stream(stream));
+}
+
+int gputils_gels_ragged_plan_set_concurrency(gputils_gels_plan_t plan, int concurrent) {
+    GPUTILS_ARG(plan != nullptr, 1, "plan is NULL");
+    GPUTILS_ARG(concurrent == 0 || concurrent == 1, 2, "concurrent must be 0 or 1");
+    plan->concurrent = concurrent != 0;
+    return GPUTILS_OK;
+}
+
+int gputils_gels_ragged_plan_get_concurrency(gputils_gels_plan_t plan, int *concurrent) {
+    GPUTILS_ARG(plan != nullptr, 1, "plan is NULL");
+    GPUTILS_ARG(concurrent != nullptr, 2, "concurrent is NULL");
+    *concurrent = plan->concurrent ? 1 : 0;
     return GPUTILS_OK;
 }
 

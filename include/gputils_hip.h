@@ -246,7 +246,8 @@ int gputils_dgels_ragged(gputils_gels_plan_t plan, const double *A, double *B, i
  * tune: the autotuner.  For every bucket it times each shape (both
  * methods, their wavefront counts, every per_cu the CU can hold) on the
  * caller's A and a device copy of B (B itself is not written), with
- * refine_steps refinements, on stream, and keeps the fastest.  It
+ * refine_steps refinements, on stream, and keeps the fastest; then it
+ * chooses between concurrent and sequential buckets (below).  It
  * allocates the copy of B and so is a plan-time call like plan_create.
  * configure and tune synchronise the device first and rewrite the plan's
  * device tables: no execution of the plan may be pending or run
@@ -255,6 +256,14 @@ int gputils_gels_ragged_plan_describe(gputils_gels_plan_t plan, int32_t *desc, i
 int gputils_gels_ragged_plan_configure(gputils_gels_plan_t plan, int bucket, int method, int waves, int per_cu);
 int gputils_gels_ragged_plan_tune(gputils_gels_plan_t plan, const double *A, const double *B, int refine_steps,
                                   void *stream);
+/* Whether a plan's buckets run concurrently (1: each bucket after the
+ * first on a stream of the plan, forked from and joined back to the
+ * caller's stream by events, so the call stays ordered on it) or one after
+ * another on the caller's stream (0).  A new plan runs them concurrently;
+ * tune times both with the tuned shapes and keeps the faster.  The results
+ * are the same either way (the buckets write disjoint systems). */
+int gputils_gels_ragged_plan_set_concurrency(gputils_gels_plan_t plan, int concurrent);
+int gputils_gels_ragged_plan_get_concurrency(gputils_gels_plan_t plan, int *concurrent);
 
 /* ------------------------------------------------------------ graphs --- */
 /* Capture the work enqueued on a (non-null) stream into an executable HIP

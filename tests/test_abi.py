@@ -56,6 +56,8 @@ def _msg():
     ("gputils_stranspose_batched", (3, 2, None, 2, 6, None, 2, 6, 1, None), -4),
     ("gputils_gels_ragged_plan_configure", (None, 0, 0, 4, 1), -1),
     ("gputils_drot", (-1, None, 1, None, 1, 1.0, 0.0, None), -1),
+    ("gputils_gels_ragged_plan_set_concurrency", (None, 1), -1),
+    ("gputils_gels_ragged_plan_get_concurrency", (None, None), -1),
     ("gputils_srot", (4, None, 0, None, 1, 1.0, 0.0, None), -3),
     ("gputils_dgivens_annihilate", (4, 3, None, 4, 0, 4, 0, None), -6),
     ("gputils_sgivens_annihilate", (4, 3, None, 4, 0, 1, 3, None), -7),

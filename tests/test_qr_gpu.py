@@ -269,6 +269,29 @@ def test_ragged_autotune_keeps_b_and_results():
     assert np.array_equal(_ragged_solve(other, tA, Bf, 1), tuned)
 
 
+def test_ragged_concurrent_and_sequential_buckets_agree():
+    """The buckets write disjoint systems: running them concurrently on the
+    plan's streams or one after another gives bit-identical results, and
+    the caller's stream is ordered after either."""
+    from gputils_amd import workloads
+    ms, ns, offA, offB, Af, Bf = workloads.ls_ragged()
+    plan = api.RaggedLeastSquares(ms, ns, offA, offB)
+    assert plan.concurrent
+    tA = dev(Af)
+    conc = _ragged_solve(plan, tA, Bf, 1)
+    plan.concurrent = False
+    assert not plan.concurrent
+    seq = _ragged_solve(plan, tA, Bf, 1)
+    assert np.array_equal(conc, seq)
+    # ordering on the caller's stream: the solve, then a read of B queued
+    # behind it, with no host synchronisation in between
+    plan.concurrent = True
+    tB = dev(Bf)
+    plan(tA, tB, refine_steps=1)
+    snap = tB.clone()
+    assert np.array_equal(snap.cpu().numpy(), conc)
+
+
 def test_ragged_plan_configure_rejects_bad_shapes():
     from gputils_amd import workloads
     ms, ns, offA, offB, Af, Bf = workloads.ls_ragged(count=64)
