@@ -52,8 +52,8 @@ def _args(argv=None):
                              "copy_calibration"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ragged-shapes", default=None,
-                    help="ls_ragged: launch shapes per bucket as METHOD:WAVES:PER_CU,... (largest bucket "
-                         "first) applied instead of autotuning; the default-shape timing is then skipped")
+                    help="ls_ragged: launch shapes per bucket as METHOD:WAVES:PER_CU:QUEUE,... (largest "
+                         "bucket first) applied instead of autotuning; the default-shape timing is then skipped")
     ap.add_argument("--ragged-concurrent", type=int, choices=[0, 1], default=None,
                     help="ls_ragged with --ragged-shapes: run the buckets concurrently (1) or in turn (0)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison runs")
@@ -426,8 +426,8 @@ def bench_ls_ragged(a):
         given = [tuple(int(v) for v in x.split(":")) for x in a.ragged_shapes.split(",")]
         if len(given) != len(default_shapes):
             raise SystemExit(f"--ragged-shapes needs {len(default_shapes)} shapes, got {len(given)}")
-        for b, (method, waves, per_cu) in enumerate(given):
-            plan.configure(b, method, waves, per_cu)
+        for b, (method, waves, per_cu, queue) in enumerate(given):
+            plan.configure(b, method, waves, per_cu, queue)
         if a.ragged_concurrent is not None:
             plan.concurrent = bool(a.ragged_concurrent)
         ev_default, t_tune, shapes, default_concurrent = None, None, plan.describe(), None
@@ -467,10 +467,11 @@ def bench_ls_ragged(a):
                                        "max_rel_err_vs_dgels_sample64": float(max(errs2))},
                  autotune={"tuned": not a.ragged_shapes, "tune_seconds": t_tune,
                            "default_shape_event_ms": ev_default,
-                           "default_shapes": [[d["maxe"], d["method"], d["waves"], d["per_cu"], d["grid"]]
+                           "default_shapes": [[d["maxe"], d["method"], d["waves"], d["per_cu"], d["grid"], d["queue"]]
                                               for d in default_shapes],
-                           "shapes": [[d["maxe"], d["method"], d["waves"], d["per_cu"], d["grid"]] for d in shapes],
-                           "shape_fields": ["maxe", "method", "waves", "per_cu", "grid"],
+                           "shapes": [[d["maxe"], d["method"], d["waves"], d["per_cu"], d["grid"], d["queue"]]
+                                      for d in shapes],
+                           "shape_fields": ["maxe", "method", "waves", "per_cu", "grid", "queue"],
                            "default_concurrent_buckets": default_concurrent, "concurrent_buckets": plan.concurrent},
                  fp_rate_tflops=flops / (ev * 1e-3) / 1e12)
 

@@ -330,15 +330,17 @@ class RaggedLeastSquares:
         size class, FP32 elements), systems, method, waves, per_cu, grid."""
         nb = ctypes.c_int(0)
         check(lib().gputils_gels_ragged_plan_describe(self._plan, None, 0, ctypes.byref(nb)), "plan_describe")
-        desc = (ctypes.c_int32 * (6 * max(nb.value, 1)))()
+        desc = (ctypes.c_int32 * (7 * max(nb.value, 1)))()
         check(lib().gputils_gels_ragged_plan_describe(self._plan, desc, nb.value, None), "plan_describe")
-        keys = ("maxe", "systems", "method", "waves", "per_cu", "grid")
-        return [dict(zip(keys, desc[6 * i:6 * i + 6])) for i in range(nb.value)]
+        keys = ("maxe", "systems", "method", "waves", "per_cu", "grid", "queue")
+        return [dict(zip(keys, desc[7 * i:7 * i + 7])) for i in range(nb.value)]
 
-    def configure(self, bucket: int, method: int, waves: int, per_cu: int):
+    def configure(self, bucket: int, method: int, waves: int, per_cu: int, queue: int = 0):
         """Set one bucket's launch shape: method 0 (column steps in LDS,
-        waves 2/4/8/16) or 1 (register panels, waves 8/16), workgroups per CU."""
-        check(lib().gputils_gels_ragged_plan_configure(self._plan, bucket, method, waves, per_cu), "plan_configure")
+        waves 2/4/8/16) or 1 (register panels, waves 8/16), workgroups per
+        CU, static ranges (queue 0) or a work queue (queue 1)."""
+        check(lib().gputils_gels_ragged_plan_configure(self._plan, bucket, method, waves, per_cu, queue),
+              "plan_configure")
 
     @property
     def concurrent(self) -> bool:

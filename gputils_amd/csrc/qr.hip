@@ -460,8 +460,9 @@ template <int MAXE, int NW, int METHOD>
 __global__ __launch_bounds__(NW * kWave) void gels_ragged_kernel(const RaggedItem *__restrict__ items,
                                                                const int *__restrict__ wstart,
                                                                const double *__restrict__ A, double *B,
-                                                               int steps) {
+                                                               int steps, int *queue) {
     __shared__ float S[MAXE + 512];  // A (m x n, m*n <= MAXE), then b (m <= 512)
+    __shared__ int s_next;
     __shared__ float s_tau[2];
     __shared__ float tau[64];
     __shared__ float c[512];
@@ -472,8 +473,21 @@ __global__ __launch_bounds__(NW * kWave) void gels_ragged_kernel(const RaggedIte
     __shared__ double bd[512];
     constexpr int NT = NW * kWave;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int it_end = wstart[blockIdx.x + 1];
-    for (int it = wstart[blockIdx.x]; it < it_end; ++it) {
+    // static: the workgroup's range wstart[w] .. wstart[w + 1] - 1.  Queue:
+    // the bucket's items wstart[0] .. wstart[grid] - 1, most expensive
+    // first, each taken by the next free workgroup (queue[0] counts them
+    // out; queue[1] counts finished workgroups, and the last one resets
+    // both, so the next launch finds them at zero)
+    const int it_end = queue ? wstart[gridDim.x] : wstart[blockIdx.x + 1];
+    int it = queue ? 0 : wstart[blockIdx.x];
+    for (;; ++it) {
+        if (queue) {
+            __syncthreads();  // the previous system's last reads of s_next are done
+            if (threadIdx.x == 0) s_next = wstart[0] + atomicAdd(queue, 1);
+            __syncthreads();
+            it = s_next;
+        }
+        if (it >= it_end) break;
         const RaggedItem w = items[it];
         const int m = w.m, n = w.n;
         const double *a = A + w.offA;
@@ -563,6 +577,13 @@ __global__ __launch_bounds__(NW * kWave) void gels_ragged_kernel(const RaggedIte
         __syncthreads();
         for (int i = threadIdx.x; i < n; i += NT) bb[i] = xd[i];
     }
+    if (queue && threadIdx.x == 0) {
+        __threadfence();  // this workgroup's last take from queue[0] precedes its count
+        if (atomicAdd(queue + 1, 1) == (int)gridDim.x - 1) {
+            atomicExch(queue, 0);
+            atomicExch(queue + 1, 0);
+        }
+    }
 }
 
 // Launch one bucket of a ragged plan: the LDS capacity MAXE, the
@@ -570,33 +591,34 @@ __global__ __launch_bounds__(NW * kWave) void gels_ragged_kernel(const RaggedIte
 // instantiation.
 template <int MAXE, int METHOD, int NW>
 void launch_ragged_one(int grid, hipStream_t st, const RaggedItem *items, const int *wstart, const double *A,
-                       double *B, int steps) {
+                       double *B, int steps, int *queue) {
     hipLaunchKernelGGL((gels_ragged_kernel<MAXE, NW, METHOD>), dim3(grid), dim3(NW * kWave), 0, st, items, wstart, A,
-                       B, steps);
+                       B, steps, queue);
 }
 
 template <int MAXE>
 void launch_ragged_maxe(int method, int waves, int grid, hipStream_t st, const RaggedItem *items, const int *wstart,
-                        const double *A, double *B, int steps) {
+                        const double *A, double *B, int steps, int *queue) {
     if (method == 1) {
-        if (waves == 16) launch_ragged_one<MAXE, 1, 16>(grid, st, items, wstart, A, B, steps);
-        else launch_ragged_one<MAXE, 1, 8>(grid, st, items, wstart, A, B, steps);
+        if (waves == 16) launch_ragged_one<MAXE, 1, 16>(grid, st, items, wstart, A, B, steps, queue);
+        else launch_ragged_one<MAXE, 1, 8>(grid, st, items, wstart, A, B, steps, queue);
     } else {
-        if (waves == 2) launch_ragged_one<MAXE, 0, 2>(grid, st, items, wstart, A, B, steps);
-        else if (waves == 4) launch_ragged_one<MAXE, 0, 4>(grid, st, items, wstart, A, B, steps);
-        else if (waves == 16) launch_ragged_one<MAXE, 0, 16>(grid, st, items, wstart, A, B, steps);
-        else launch_ragged_one<MAXE, 0, 8>(grid, st, items, wstart, A, B, steps);
+        if (waves == 2) launch_ragged_one<MAXE, 0, 2>(grid, st, items, wstart, A, B, steps, queue);
+        else if (waves == 4) launch_ragged_one<MAXE, 0, 4>(grid, st, items, wstart, A, B, steps, queue);
+        else if (waves == 16) launch_ragged_one<MAXE, 0, 16>(grid, st, items, wstart, A, B, steps, queue);
+        else launch_ragged_one<MAXE, 0, 8>(grid, st, items, wstart, A, B, steps, queue);
     }
 }
 
+// queue: NULL for the static ranges, else the bucket's two counters
 int launch_ragged(int maxe, int method, int waves, int grid, hipStream_t st, const RaggedItem *items,
-                  const int *wstart, const double *A, double *B, int steps) {
+                  const int *wstart, const double *A, double *B, int steps, int *queue) {
     switch (maxe) {
-    case 2048: launch_ragged_maxe<2048>(method, waves, grid, st, items, wstart, A, B, steps); break;
-    case 4096: launch_ragged_maxe<4096>(method, waves, grid, st, items, wstart, A, B, steps); break;
-    case 8192: launch_ragged_maxe<8192>(method, waves, grid, st, items, wstart, A, B, steps); break;
-    case 16384: launch_ragged_maxe<16384>(method, waves, grid, st, items, wstart, A, B, steps); break;
-    default: launch_ragged_maxe<32768>(method, waves, grid, st, items, wstart, A, B, steps); break;
+    case 2048: launch_ragged_maxe<2048>(method, waves, grid, st, items, wstart, A, B, steps, queue); break;
+    case 4096: launch_ragged_maxe<4096>(method, waves, grid, st, items, wstart, A, B, steps, queue); break;
+    case 8192: launch_ragged_maxe<8192>(method, waves, grid, st, items, wstart, A, B, steps, queue); break;
+    case 16384: launch_ragged_maxe<16384>(method, waves, grid, st, items, wstart, A, B, steps, queue); break;
+    default: launch_ragged_maxe<32768>(method, waves, grid, st, items, wstart, A, B, steps, queue); break;
     }
     return launch_check("gels_ragged_kernel");
 }
@@ -659,12 +681,14 @@ struct gputils_gels_plan {
         int waves;         // wavefronts per workgroup: 2, 4, 8, 16 (method 0) or 8, 16 (method 1)
         int per_cu;        // workgroups per CU the grid is sized for
         int grid;          // min(count, CUs * per_cu)
+        bool queue;        // systems taken from a work queue, else static LPT ranges
         std::vector<int> order;  // the bucket's systems, most expensive first
     };
     int64_t count = 0;
     int cus = 256;
     qr::RaggedItem *d_items = nullptr;
     int *d_wstart = nullptr;  // per bucket: grid + 1 absolute item offsets
+    int *d_queue = nullptr;   // per bucket: the work queue's two counters (zero between launches)
     std::vector<int32_t> m, n;
     std::vector<int64_t> offA, offB;
     std::vector<Bucket> buckets;
@@ -685,9 +709,9 @@ constexpr int kRaggedBucketMax[] = {2048, 4096, 8192, 16384, 32768};
 constexpr int kRaggedBuckets = 5;
 
 // LDS of gels_ragged_kernel<MAXE, *>: the FP32 matrix and b column plus
-// the fixed per-system vectors (s_tau 8, tau 256, c 2048, h 256, dx 256,
-// xd 512, rd 4096, bd 4096 bytes)
-int ragged_lds_bytes(int maxe) { return (maxe + 512) * 4 + 11528; }
+// the fixed per-system vectors (s_next 4 and padding 4, s_tau 8, tau 256,
+// c 2048, h 256, dx 256, xd 512, rd 4096, bd 4096 bytes)
+int ragged_lds_bytes(int maxe) { return (maxe + 512) * 4 + 11536; }
 
 // most workgroups of one bucket that a CU can hold at once: LDS (160 KiB)
 // and 32 wavefronts per CU
@@ -703,6 +727,20 @@ double ragged_cost(int m, int n) { return (double)m * n * n / 64.0 + 2000.0 * n;
 // each system, most expensive first, goes to the least-loaded workgroup)
 // and upload the items and range boundaries.
 int ragged_upload_bucket(gputils_gels_plan *p, const gputils_gels_plan::Bucket &bk) {
+    if (bk.queue) {
+        // the bucket's systems most expensive first; only the first and the
+        // last range boundary are read
+        std::vector<qr::RaggedItem> items;
+        items.reserve((size_t)bk.count);
+        for (int o : bk.order) items.push_back({p->m[o], p->n[o], p->offA[o], p->offB[o]});
+        std::vector<int> wstart((size_t)bk.grid + 1, bk.begin);
+        wstart.back() = bk.begin + bk.count;
+        GPUTILS_HIP(hipMemcpy(p->d_items + bk.begin, items.data(), sizeof(qr::RaggedItem) * items.size(),
+                              hipMemcpyHostToDevice));
+        GPUTILS_HIP(hipMemcpy(p->d_wstart + bk.wbase, wstart.data(), sizeof(int) * wstart.size(),
+                              hipMemcpyHostToDevice));
+        return GPUTILS_OK;
+    }
     std::vector<std::vector<int>> bins((size_t)bk.grid);
     std::priority_queue<std::pair<double, int>, std::vector<std::pair<double, int>>,
                         std::greater<std::pair<double, int>>> heap;
@@ -742,7 +780,7 @@ int ragged_execute(gputils_gels_plan *plan, const double *A, double *B, int step
         hipStream_t sb = (conc && b > 0) ? plan->streams[b - 1] : st;
         if (conc && b > 0) GPUTILS_HIP(hipStreamWaitEvent(sb, plan->fork, 0));
         int rc = qr::launch_ragged(bk.maxe, bk.method, bk.waves, bk.grid, sb, plan->d_items,
-                                   plan->d_wstart + bk.wbase, A, B, steps);
+                                   plan->d_wstart + bk.wbase, A, B, steps, bk.queue ? plan->d_queue + 2 * b : nullptr);
         if (rc) return rc;
         if (conc && b > 0) GPUTILS_HIP(hipEventRecord(plan->done[b - 1], sb));
     }
@@ -751,7 +789,9 @@ int ragged_execute(gputils_gels_plan *plan, const double *A, double *B, int step
     return GPUTILS_OK;
 }
 
-void ragged_set_shape(const gputils_gels_plan *p, gputils_gels_plan::Bucket &bk, int method, int waves, int per_cu) {
+void ragged_set_shape(const gputils_gels_plan *p, gputils_gels_plan::Bucket &bk, int method, int waves, int per_cu,
+                      bool queue) {
+    bk.queue = queue;
     bk.method = method;
     bk.waves = waves;
     bk.per_cu = per_cu;
@@ -892,7 +932,7 @@ int gputils_gels_ragged_plan_create(gputils_gels_plan_t *plan, int64_t count, co
         b.order = std::move(mem);
         // untuned shape: column steps, eight wavefronts, as many workgroups
         // per CU as fit
-        ragged_set_shape(p, b, 0, 8, ragged_max_per_cu(b.maxe, 8));
+        ragged_set_shape(p, b, 0, 8, ragged_max_per_cu(b.maxe, 8), false);
         begin += b.count;
         p->buckets.push_back(std::move(b));
     }
@@ -916,6 +956,8 @@ int gputils_gels_ragged_plan_create(gputils_gels_plan_t *plan, int64_t count, co
     if (count > 0) {
         hipError_t e = hipMalloc(&p->d_items, sizeof(qr::RaggedItem) * (size_t)count);
         if (e == hipSuccess) e = hipMalloc(&p->d_wstart, sizeof(int) * ((size_t)count + p->buckets.size()));
+        if (e == hipSuccess) e = hipMalloc(&p->d_queue, sizeof(int) * 2 * p->buckets.size());
+        if (e == hipSuccess) e = hipMemset(p->d_queue, 0, sizeof(int) * 2 * p->buckets.size());
         int rc = e == hipSuccess ? GPUTILS_OK : hip_fail(e, "plan allocation");
         for (size_t i = 0; rc == GPUTILS_OK && i < p->buckets.size(); ++i) rc = ragged_upload_bucket(p, p->buckets[i]);
         if (rc != GPUTILS_OK) {
@@ -935,6 +977,7 @@ int gputils_gels_ragged_plan_destroy(gputils_gels_plan_t plan) {
     };
     if (plan->d_items) keep(hipFree(plan->d_items));
     if (plan->d_wstart) keep(hipFree(plan->d_wstart));
+    if (plan->d_queue) keep(hipFree(plan->d_queue));
     for (hipStream_t sb : plan->streams) keep(hipStreamDestroy(sb));
     for (hipEvent_t eb : plan->done) keep(hipEventDestroy(eb));
     if (plan->fork) keep(hipEventDestroy(plan->fork));
@@ -949,23 +992,25 @@ int gputils_gels_ragged_plan_describe(gputils_gels_plan_t plan, int32_t *desc, i
     if (nbuckets) *nbuckets = (int)plan->buckets.size();
     for (int i = 0; i < capacity && i < (int)plan->buckets.size(); ++i) {
         const auto &bk = plan->buckets[(size_t)i];
-        const int32_t v[6] = {bk.maxe, bk.count, bk.method, bk.waves, bk.per_cu, bk.grid};
-        for (int q = 0; q < 6; ++q) desc[6 * i + q] = v[q];
+        const int32_t v[7] = {bk.maxe, bk.count, bk.method, bk.waves, bk.per_cu, bk.grid, bk.queue ? 1 : 0};
+        for (int q = 0; q < 7; ++q) desc[7 * i + q] = v[q];
     }
     return GPUTILS_OK;
 }
 
-int gputils_gels_ragged_plan_configure(gputils_gels_plan_t plan, int bucket, int method, int waves, int per_cu) {
+int gputils_gels_ragged_plan_configure(gputils_gels_plan_t plan, int bucket, int method, int waves, int per_cu,
+                                       int queue) {
     GPUTILS_ARG(plan != nullptr, 1, "plan is NULL");
     GPUTILS_ARG(bucket >= 0 && bucket < (int)plan->buckets.size(), 2, "bucket out of range");
     GPUTILS_ARG(method == 0 || method == 1, 3, "method must be 0 or 1");
     GPUTILS_ARG(method == 0 ? (waves == 2 || waves == 4 || waves == 8 || waves == 16) : (waves == 8 || waves == 16), 4,
                 "waves must be 2, 4, 8 or 16 (method 0) or 8 or 16 (method 1)");
     GPUTILS_ARG(per_cu >= 1 && per_cu <= 64, 5, "per_cu must be in [1, 64]");
+    GPUTILS_ARG(queue == 0 || queue == 1, 6, "queue must be 0 or 1");
     // the device tables change: no execution of the plan may be pending
     GPUTILS_HIP(hipDeviceSynchronize());
     auto &bk = plan->buckets[(size_t)bucket];
-    ragged_set_shape(plan, bk, method, waves, per_cu);
+    ragged_set_shape(plan, bk, method, waves, per_cu, queue != 0);
     return ragged_upload_bucket(plan, bk);
 }
 
@@ -990,15 +1035,17 @@ int gputils_gels_ragged_plan_tune(gputils_gels_plan_t plan, const double *A, con
     for (size_t bi = 0; rc == GPUTILS_OK && bi < plan->buckets.size(); ++bi) {
         auto &bk = plan->buckets[bi];
         int best_m = bk.method, best_w = bk.waves, best_pc = bk.per_cu;
+        bool best_q = bk.queue;
         float best_ms = -1.0f;
-        std::vector<std::pair<int, int>> seen;  // (method * 64 + waves, grid) already timed
+        std::vector<std::pair<int, int>> seen;  // (queue * 128 + method * 64 + waves, grid) already timed
         static const int kShapes[][2] = {{0, 2}, {0, 4}, {0, 8}, {0, 16}, {1, 8}, {1, 16}};
+        for (int qmode = 0; qmode < 2; ++qmode)
         for (const auto &sh : kShapes) {
             const int method = sh[0], waves = sh[1];
             const int cap = ragged_max_per_cu(bk.maxe, waves);
             for (int pc = 1; pc <= cap && rc == GPUTILS_OK; ++pc) {
-                ragged_set_shape(plan, bk, method, waves, pc);
-                const auto key = std::make_pair(method * 64 + waves, bk.grid);
+                ragged_set_shape(plan, bk, method, waves, pc, qmode == 1);
+                const auto key = std::make_pair(qmode * 128 + method * 64 + waves, bk.grid);
                 if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
                 seen.push_back(key);
                 rc = ragged_upload_bucket(plan, bk);
@@ -1011,7 +1058,8 @@ int gputils_gels_ragged_plan_tune(gputils_gels_plan_t plan, const double *A, con
                         break;
                     }
                     rc = qr::launch_ragged(bk.maxe, bk.method, bk.waves, bk.grid, st, plan->d_items,
-                                           plan->d_wstart + bk.wbase, A, scratch, refine_steps);
+                                           plan->d_wstart + bk.wbase, A, scratch, refine_steps,
+                                           bk.queue ? plan->d_queue + 2 * bi : nullptr);
                     if (rc != GPUTILS_OK) break;
                     e = hipEventRecord(ev1, st);
                     if (e == hipSuccess) e = hipEventSynchronize(ev1);
@@ -1028,10 +1076,11 @@ int gputils_gels_ragged_plan_tune(gputils_gels_plan_t plan, const double *A, con
                     best_m = method;
                     best_w = waves;
                     best_pc = pc;
+                    best_q = qmode == 1;
                 }
             }
         }
-        ragged_set_shape(plan, bk, best_m, best_w, best_pc);
+        ragged_set_shape(plan, bk, best_m, best_w, best_pc, best_q);
         if (rc == GPUTILS_OK) rc = ragged_upload_bucket(plan, bk);
     }
     // with the shapes fixed, the whole plan concurrently and in turn

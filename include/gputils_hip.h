@@ -231,20 +231,24 @@ int gputils_dgels_ragged(gputils_gels_plan_t plan, const double *A, double *B, i
  * factorised column by column, one barrier per column; 1: panels of
  * 64 / waves columns held in registers, one barrier per panel), the
  * wavefronts per workgroup (2, 4, 8 or 16 for method 0; 8 or 16 for
- * method 1) and the workgroups per CU (the grid is min(systems in the
- * bucket, CUs * per_cu)); the systems are assigned to the grid's
- * workgroups longest-processing-time first.  A new plan uses method 0, 8
- * wavefronts and as many workgroups per CU as fit (LDS, 32 wavefronts per
- * CU).  For one method every shape gives bit-identical results; the two
+ * method 1), the workgroups per CU (the grid is min(systems in the
+ * bucket, CUs * per_cu)) and the scheduling: static (queue = 0: the
+ * systems are assigned to the grid's workgroups longest-processing-time
+ * first by a cost model) or a work queue (queue = 1: each workgroup takes
+ * the next system, most expensive first, from a device counter that the
+ * last workgroup resets, so nothing is cleared between launches).  A new
+ * plan uses method 0, 8 wavefronts, as many workgroups per CU as fit (LDS,
+ * 32 wavefronts per CU) and static scheduling.  For one method every shape gives bit-identical results; the two
  * methods round the FP32 factorisation differently (the refined solutions
  * agree to the refinement's accuracy).
  *
- * describe: writes {maxe, systems, method, waves, per_cu, grid} for the
- * first min(capacity, buckets) buckets to desc (6 int32 each) and the
+ * describe: writes {maxe, systems, method, waves, per_cu, grid, queue} for
+ * the first min(capacity, buckets) buckets to desc (7 int32 each) and the
  * bucket count to *nbuckets (if not NULL).
  * configure: sets one bucket's shape.
  * tune: the autotuner.  For every bucket it times each shape (both
- * methods, their wavefront counts, every per_cu the CU can hold) on the
+ * methods, their wavefront counts, every per_cu the CU can hold, both
+ * schedulings) on the
  * caller's A and a device copy of B (B itself is not written), with
  * refine_steps refinements, on stream, and keeps the fastest; then it
  * chooses between concurrent and sequential buckets (below).  It
@@ -253,7 +257,8 @@ int gputils_dgels_ragged(gputils_gels_plan_t plan, const double *A, double *B, i
  * device tables: no execution of the plan may be pending or run
  * concurrently. */
 int gputils_gels_ragged_plan_describe(gputils_gels_plan_t plan, int32_t *desc, int capacity, int *nbuckets);
-int gputils_gels_ragged_plan_configure(gputils_gels_plan_t plan, int bucket, int method, int waves, int per_cu);
+int gputils_gels_ragged_plan_configure(gputils_gels_plan_t plan, int bucket, int method, int waves, int per_cu,
+                                       int queue);
 int gputils_gels_ragged_plan_tune(gputils_gels_plan_t plan, const double *A, const double *B, int refine_steps,
                                   void *stream);
 /* Whether a plan's buckets run concurrently (1: each bucket after the

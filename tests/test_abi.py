@@ -54,7 +54,7 @@ def _msg():
     ("gputils_dgemm_strided_batched", (2, 2, 2, 1.0, None, 2, 4, None, 2, 4, 0.0, None, 1, 4, 1, None), -13),
     ("gputils_dnrm2", (-5, None, None, None, None), -1),
     ("gputils_stranspose_batched", (3, 2, None, 2, 6, None, 2, 6, 1, None), -4),
-    ("gputils_gels_ragged_plan_configure", (None, 0, 0, 4, 1), -1),
+    ("gputils_gels_ragged_plan_configure", (None, 0, 0, 4, 1, 0), -1),
     ("gputils_drot", (-1, None, 1, None, 1, 1.0, 0.0, None), -1),
     ("gputils_gels_ragged_plan_set_concurrency", (None, 1), -1),
     ("gputils_gels_ragged_plan_get_concurrency", (None, None), -1),
@@ -91,9 +91,9 @@ def test_empty_ragged_plan_shape_calls():
         assert L.gputils_gels_ragged_plan_describe(plan, None, 0, ctypes.byref(nb)) == 0
         assert nb.value == 0
         assert L.gputils_gels_ragged_plan_tune(plan, None, None, 1, None) == 0
-        assert L.gputils_gels_ragged_plan_configure(plan, 0, 0, 8, 1) == -2
+        assert L.gputils_gels_ragged_plan_configure(plan, 0, 0, 8, 1, 0) == -2
         assert "invalid argument 2" in _msg()
-        assert L.gputils_gels_ragged_plan_configure(plan, 0, 5, 3, 1) == -2  # the bucket is checked first
+        assert L.gputils_gels_ragged_plan_configure(plan, 0, 5, 3, 1, 7) == -2  # the bucket is checked first
     finally:
         assert L.gputils_gels_ragged_plan_destroy(plan) == 0
 

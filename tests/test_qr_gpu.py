@@ -193,7 +193,8 @@ def test_ragged_plan_buckets_and_default_shape():
     desc = plan.describe()
     assert [(d["maxe"], d["systems"]) for d in desc] == _bucket_sizes(ms, ns)
     for d in desc:
-        assert d["method"] == 0 and d["waves"] == 8 and d["per_cu"] >= 1 and 1 <= d["grid"] <= d["systems"]
+        assert d["method"] == 0 and d["waves"] == 8 and d["queue"] == 0
+        assert d["per_cu"] >= 1 and 1 <= d["grid"] <= d["systems"]
 
 
 _SHAPES = {0: (2, 4, 8, 16), 1: (8, 16)}
@@ -202,27 +203,31 @@ _SHAPES = {0: (2, 4, 8, 16), 1: (8, 16)}
 @pytest.mark.parametrize("method", [0, 1])
 def test_ragged_every_launch_shape_gives_identical_results(method):
     """For one factorisation method the launch shape (wavefronts per
-    workgroup, workgroups per CU and so the assignment of systems to
-    workgroups) changes no system's arithmetic: every shape reproduces the
-    method's first shape bit for bit, also where a workgroup solves several
-    systems in turn (per_cu = 1)."""
+    workgroup, workgroups per CU, static ranges or the work queue, and so
+    the assignment of systems to workgroups) changes no system's
+    arithmetic: every shape reproduces the method's first shape bit for
+    bit, also where a workgroup solves several systems in turn (per_cu = 1),
+    and repeated queue launches (the counters reset themselves) agree."""
     from gputils_amd import workloads
     ms, ns, offA, offB, Af, Bf = workloads.ls_ragged()
     plan = api.RaggedLeastSquares(ms, ns, offA, offB)
     tA = dev(Af)
     nb = len(plan.describe())
     ref = None
-    for waves in _SHAPES[method]:
-        for per_cu in (1, 2, 3):
-            for b in range(nb):
-                plan.configure(b, method, waves, per_cu)
-            for d in plan.describe():
-                assert (d["method"], d["waves"], d["per_cu"]) == (method, waves, per_cu)
-                assert 1 <= d["grid"] <= d["systems"]
-            got = _ragged_solve(plan, tA, Bf, 1)
-            if ref is None:
-                ref = got
-            assert np.array_equal(got, ref), (waves, per_cu)
+    for queue in (0, 1):
+        for waves in _SHAPES[method]:
+            for per_cu in (1, 2, 3):
+                for b in range(nb):
+                    plan.configure(b, method, waves, per_cu, queue)
+                for d in plan.describe():
+                    assert (d["method"], d["waves"], d["per_cu"], d["queue"]) == (method, waves, per_cu, queue)
+                    assert 1 <= d["grid"] <= d["systems"]
+                got = _ragged_solve(plan, tA, Bf, 1)
+                if ref is None:
+                    ref = got
+                assert np.array_equal(got, ref), (queue, waves, per_cu)
+                if queue:
+                    assert np.array_equal(_ragged_solve(plan, tA, Bf, 1), ref), ("repeat", waves, per_cu)
 
 
 def test_ragged_register_panel_method_matches_oracle(oracle):
@@ -262,10 +267,11 @@ def test_ragged_autotune_keeps_b_and_results():
     assert [(d["maxe"], d["systems"]) for d in desc] == _bucket_sizes(ms, ns)
     for d in desc:
         assert d["waves"] in _SHAPES[d["method"]] and d["per_cu"] >= 1 and 1 <= d["grid"] <= d["systems"]
+        assert d["queue"] in (0, 1)
     tuned = _ragged_solve(plan, tA, Bf, 1)
     other = api.RaggedLeastSquares(ms, ns, offA, offB)
     for b, d in enumerate(desc):
-        other.configure(b, d["method"], d["waves"], d["per_cu"])
+        other.configure(b, d["method"], d["waves"], d["per_cu"], d["queue"])
     assert np.array_equal(_ragged_solve(other, tA, Bf, 1), tuned)
 
 
@@ -306,6 +312,8 @@ def test_ragged_plan_configure_rejects_bad_shapes():
         plan.configure(0, 0, 4, 0)    # per_cu >= 1
     with pytest.raises(api.GputilsError):
         plan.configure(len(plan.describe()), 0, 4, 1)  # no such bucket
+    with pytest.raises(api.GputilsError):
+        plan.configure(0, 0, 4, 1, 2)  # queue must be 0 or 1
 
 
 def test_ragged_plan_rejects_bad_sizes():
