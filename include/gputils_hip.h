@@ -255,7 +255,9 @@ int gputils_dgels_ragged(gputils_gels_plan_t plan, const double *A, double *B, i
  * allocates the copy of B and so is a plan-time call like plan_create.
  * configure and tune synchronise the device first and rewrite the plan's
  * device tables: no execution of the plan may be pending or run
- * concurrently. */
+ * concurrently.  While a bucket uses the work queue, executions of one
+ * plan must not overlap each other either (they would share its counters;
+ * executions on one stream never overlap). */
 int gputils_gels_ragged_plan_describe(gputils_gels_plan_t plan, int32_t *desc, int capacity, int *nbuckets);
 int gputils_gels_ragged_plan_configure(gputils_gels_plan_t plan, int bucket, int method, int waves, int per_cu,
                                        int queue);
