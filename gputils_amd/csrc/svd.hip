@@ -174,12 +174,27 @@ __global__ __launch_bounds__(kThreads) void gesvd_kernel(int m, int n, const T *
                 tournament_pair(grp, r, n1, p, q);
                 if (q < n) {
                     double *wp = W + p * n, *wq = W + q * n;
+                    // the lane's entries i = gl + 8 u of both columns, loaded
+                    // together (indices clamped into the column, zero past n,
+                    // which adds nothing below) and kept for the rotation
+                    constexpr int kPer = kMaxN / 8;
+                    double xw[kPer], yw[kPer];
+#pragma unroll
+                    for (int u = 0; u < kPer; ++u) {
+                        const int i = gl + 8 * u, ic = i < n ? i : n - 1;
+                        xw[u] = wp[ic];
+                        yw[u] = wq[ic];
+                    }
+#pragma unroll
+                    for (int u = 0; u < kPer; ++u) {
+                        if (gl + 8 * u >= n) xw[u] = yw[u] = 0.0;
+                    }
                     double al = 0.0, be = 0.0, ga = 0.0;
-                    for (int i = gl; i < n; i += 8) {
-                        const double x = wp[i], y = wq[i];
-                        al = fma(x, x, al);
-                        be = fma(y, y, be);
-                        ga = fma(x, y, ga);
+#pragma unroll
+                    for (int u = 0; u < kPer; ++u) {
+                        al = fma(xw[u], xw[u], al);
+                        be = fma(yw[u], yw[u], be);
+                        ga = fma(xw[u], yw[u], ga);
                     }
                     al = group8_sum(al);
                     be = group8_sum(be);
@@ -192,23 +207,40 @@ __global__ __launch_bounds__(kThreads) void gesvd_kernel(int m, int n, const T *
                         // 2 |ga|: one square root and one division
                         const double d = be - al, g2 = 2.0 * fabs(ga);
                         const double sg = (d == 0.0 || ((d > 0.0) == (ga > 0.0))) ? 1.0 : -1.0;
-                        const double t = sg * g2 / (fabs(d) + sqrt(fma(d, d, g2 * g2)));
+                        // the root and the quotient from the hardware
+                        // estimates with one Newton step each: t only sets
+                        // the angle (cs, sn below form a rotation for any
+                        // t), and these are accurate to a few ulp
+                        const double h = fma(d, d, g2 * g2);
+                        double rh = __builtin_amdgcn_rsq(h);
+                        rh = rh * fma(-0.5 * h, rh * rh, 1.5);
+                        const double den = fma(h, rh, fabs(d));  // |d| + sqrt(h)
+                        double inv = __builtin_amdgcn_rcp(den);
+                        inv = inv * fma(-den, inv, 2.0);
+                        const double t = sg * g2 * inv;
                         // cs = 1 / sqrt(1 + t^2): hardware estimate and one
                         // Newton step (relative error ~ the estimate's squared)
                         const double t2p1 = fma(t, t, 1.0);
                         double cs = __builtin_amdgcn_rsq(t2p1);
                         cs = cs * fma(-0.5 * t2p1, cs * cs, 1.5);
                         const double sn = cs * t;
-                        for (int i = gl; i < n; i += 8) {
-                            const double x = wp[i], y = wq[i];
-                            wp[i] = cs * x - sn * y;
-                            wq[i] = sn * x + cs * y;
-                        }
                         double *vp = X + p * n, *vq = X + q * n;
-                        for (int i = gl; i < n; i += 8) {
-                            const double x = vp[i], y = vq[i];
-                            vp[i] = cs * x - sn * y;
-                            vq[i] = sn * x + cs * y;
+                        double xv[kPer], yv[kPer];
+#pragma unroll
+                        for (int u = 0; u < kPer; ++u) {
+                            const int i = gl + 8 * u, ic = i < n ? i : n - 1;
+                            xv[u] = vp[ic];
+                            yv[u] = vq[ic];
+                        }
+#pragma unroll
+                        for (int u = 0; u < kPer; ++u) {
+                            const int i = gl + 8 * u;
+                            if (i < n) {
+                                wp[i] = cs * xw[u] - sn * yw[u];
+                                wq[i] = sn * xw[u] + cs * yw[u];
+                                vp[i] = cs * xv[u] - sn * yv[u];
+                                vq[i] = sn * xv[u] + cs * yv[u];
+                            }
                         }
                         rot = 1;
                     }
