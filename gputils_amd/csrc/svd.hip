@@ -22,8 +22,9 @@
 //      (<= m eps sigma_max) v_j keeps its sign and u_j is the first unit
 //      vector e_k whose residual against the columns already placed exceeds
 //      1/2 (two Gram-Schmidt passes), as LAPACK's bidiagonal QR leaves it.
-//   4. U = Q diag(U_R, I_{m-n}), formed up to 16 columns at a time with one
-//      16-lane group per column.
+//   4. U = Q diag(U_R, I_{m-n}): for m <= 128 all columns at once in
+//      registers (eight per 16-lane group, eight rows per lane); otherwise up
+//      to 16 columns at a time with one 16-lane group per column.
 #include <algorithm>
 #include <cfloat>
 
@@ -354,9 +355,66 @@ __global__ __launch_bounds__(kThreads) void gesvd_kernel(int m, int n, const T *
     }
     __syncthreads();
 
-    // 4. U = Q diag(U_R, I): chunk columns in W, one 16-lane group each
+    // 4. U = Q diag(U_R, I)
     const int g16 = tid >> 4, l16 = tid & 15;
     T *u = U + b * strideU;
+    if (m <= 128) {
+        // every column at once in registers: 16-lane group g16 holds
+        // columns g16 + 16 q, lane l16 rows l16 + 16 k; each reflector is
+        // read once (rows clamped into the column, masked to [0..0, 1, v])
+        // and its eight column updates reduce together
+        constexpr int kQ = 8, kK = 8;
+        double xr[kQ][kK];
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) {
+            const int cc = g16 + 16 * q;
+#pragma unroll
+            for (int k = 0; k < kK; ++k) {
+                const int i = l16 + 16 * k;
+                const int xi = (i < n && cc < n) ? i + cc * n : 0;
+                const double xv = X[xi];
+                xr[q][k] = (cc < n) ? ((i < n) ? xv : 0.0) : ((i == cc) ? 1.0 : 0.0);
+            }
+        }
+        for (int j = n - 1; j >= 0; --j) {
+            const double t = tau[j];
+            if (t == 0.0) continue;  // uniform
+            const double *vcol = G + (int64_t)j * m;
+            double vr[kK];
+#pragma unroll
+            for (int k = 0; k < kK; ++k) {
+                const int i = l16 + 16 * k;
+                const double e = vcol[i < m ? i : m - 1];
+                vr[k] = (i > j && i < m) ? e : ((i == j) ? 1.0 : 0.0);
+            }
+            double w[kQ];
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) {
+                w[q] = 0.0;
+#pragma unroll
+                for (int k = 0; k < kK; ++k) w[q] = fma(vr[k], xr[q][k], w[q]);
+            }
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) w[q] = group16_sum(w[q]);
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) {
+                const double tw = t * w[q];
+#pragma unroll
+                for (int k = 0; k < kK; ++k) xr[q][k] = fma(-tw, vr[k], xr[q][k]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) {
+            const int cc = g16 + 16 * q;
+#pragma unroll
+            for (int k = 0; k < kK; ++k) {
+                const int i = l16 + 16 * k;
+                if (cc < m && i < m) u[i + (int64_t)cc * ldu] = (T)xr[q][k];
+            }
+        }
+        return;
+    }
+    // larger m: chunks of columns in W, one 16-lane group each
     for (int c0 = 0; c0 < m; c0 += L.ch) {
         const int cc = c0 + g16;
         if (g16 < L.ch && cc < m) {
