@@ -69,8 +69,8 @@ def test_rank_golden(dtype, eps):
     assert api.svd_rank_batched(S, eps).cpu().tolist() == G.RANK_EXPECTED
 
 
-SHAPES = [(1, 1), (5, 1), (3, 2), (10, 4), (17, 9), (30, 30), (64, 64), (96, 48), (200, 50), (300, 64),
-          (1000, 20)]
+SHAPES = [(1, 1), (5, 1), (3, 2), (10, 4), (17, 9), (30, 30), (64, 64), (96, 48), (128, 64), (129, 40),
+          (200, 50), (300, 64), (1000, 20)]  # U in registers up to m = 128, in column chunks beyond
 
 
 @pytest.mark.parametrize("m,n", SHAPES)
