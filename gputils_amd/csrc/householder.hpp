@@ -219,7 +219,11 @@ __device__ __forceinline__ void hqr_factor(T *A, int64_t lda, int m, int n, T *t
         const T t = s_tau[j & 1];
         if (j + 1 < ntot && wave == ((j + 1) & (NW - 1))) {
             if (j + 1 < n && m <= 8 * kWave) {
-                const T t1 = update_and_reflect<T, ACC, DOT, 8>(A, lda, m, j, t);
+                // as many row groups as the column has (the terms past m
+                // are zeros either way)
+                const T t1 = (m <= 2 * kWave)   ? update_and_reflect<T, ACC, DOT, 2>(A, lda, m, j, t)
+                             : (m <= 4 * kWave) ? update_and_reflect<T, ACC, DOT, 4>(A, lda, m, j, t)
+                                                : update_and_reflect<T, ACC, DOT, 8>(A, lda, m, j, t);
                 if (lane == 0) {
                     s_tau[(j + 1) & 1] = t1;
                     tau[j + 1] = t1;
