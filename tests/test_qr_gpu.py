@@ -298,6 +298,40 @@ def test_ragged_concurrent_and_sequential_buckets_agree():
     assert np.array_equal(snap.cpu().numpy(), conc)
 
 
+@pytest.mark.parametrize("queue,concurrent", [(0, False), (1, False), (0, True), (1, True)])
+def test_ragged_plan_graph_replays_match_eager(queue, concurrent):
+    """One plan execution captured in a HIP graph (the concurrent buckets
+    fork from and join back to the capturing stream by events; the work
+    queue's counters reset themselves, no memset node) and replayed with
+    eager work between replays: every replay equals the eager solve."""
+    from gputils_amd import workloads
+    ms, ns, offA, offB, Af, Bf = workloads.ls_ragged(seed=9, count=1024)
+    plan = api.RaggedLeastSquares(ms, ns, offA, offB)
+    for b in range(len(plan.describe())):
+        plan.configure(b, 0, 8, 2, queue)
+    plan.concurrent = concurrent
+    tA = dev(Af)
+    eager = _ragged_solve(plan, tA, Bf, 1)
+    B0 = dev(Bf)
+    tB = B0.clone()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        plan(tA, tB, refine_steps=1)  # warm-up outside the capture
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g, stream=s):
+            plan(tA, tB, refine_steps=1)
+    torch.cuda.synchronize()
+    for rep in range(4):
+        tB.copy_(B0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(tB.cpu().numpy(), eager), rep
+        _ = (tA[:4096] * 2).sum().item()  # eager work between replays
+
+
 def test_ragged_plan_configure_rejects_bad_shapes():
     from gputils_amd import workloads
     ms, ns, offA, offB, Af, Bf = workloads.ls_ragged(count=64)
