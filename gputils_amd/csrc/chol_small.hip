@@ -7,11 +7,18 @@
 // factor and the solve are fused.  Right-looking unblocked algorithm: at
 // step k every lane writes its current A[i][k] to a per-half LDS column
 // buffer and reads the pivot and the column back (broadcast reads), then
-// updates its row with A[i][j] -= A[i][k] A[j][k] / A[k][k].  The matrix is
-// padded with the identity to N = 16 or 32 rows so the unrolled code needs
-// no bounds checks.  The forward solve uses the row layout; the backward
-// solve (L^T x = y) reads L by columns from an LDS copy.  Both multiply by
-// the reciprocal pivots kept from the factorisation instead of dividing.
+// updates its row with A[i][j] -= A[i][k] A[j][k] / A[k][k].  Only the lower
+// triangle is used: the entries right of the diagonal, loaded with the row,
+// take part in the row updates but never reach the lower part, and each is replaced by zero when
+// its column is reached, so the registers end with the strictly lower L and
+// the diagonal and the reciprocal pivots sit in LDS.  Rows past n are zero
+// rows whose pivots are taken as 1 (identity padding to N = 16 or 32) so the
+// unrolled code needs no bounds checks and no per-element lane masks.  The
+// solves keep every lane's value unscaled and publish the scaled one
+// (value * 1 / L(k, k)) at step k, which is the same product the scaled
+// form computes; the backward solve reads L(k, i) from a copy of the
+// strictly lower L packed by rows in LDS (4 KB per matrix for n = 32, so a
+// CU's LDS holds sixteen workgroups).
 #include "chol_small.hpp"
 #include "common.hpp"
 
@@ -28,28 +35,34 @@ __device__ __forceinline__ void static_for(F &&f) {
     }
 }
 
-constexpr int kLdT = 33;  // LDS copy of L: element (row i, column j) at j * kLdT + i
+// a copy of v the compiler cannot relate to v: comparisons against it are
+// recomputed where they are used instead of being kept live across phases
+__device__ __forceinline__ int opaque(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
 
 template <int N, bool WRITE_L>
 __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double *A, int64_t lda, int64_t strideA,
                                                         const double *B, int64_t ldb, int64_t strideB, double *X,
                                                         int64_t ldx, int64_t strideX, int *info, int64_t batch,
                                                         const int *mask) {
-    __shared__ __attribute__((aligned(16))) double LT[2][N * kLdT];
-    __shared__ __attribute__((aligned(16))) double CB[2][N];  // per-half broadcast buffer
-    __shared__ double RI[2][N];                               // per-half 1 / L(k, k)
+    constexpr int kTri = N * (N - 1) / 2;
+    __shared__ __attribute__((aligned(16))) double LR[2][kTri];  // strict lower L, packed by rows
+    __shared__ __attribute__((aligned(16))) double CB[2][N];     // per-half broadcast buffer
+    __shared__ double RI[2][N];                                   // per-half 1 / L(k, k)
+    __shared__ double SD[2][N];                                   // per-half L(k, k)
     const int lane = threadIdx.x, i = lane & 31;
     const bool hi = lane >= 32;
     const int64_t b = 2 * (int64_t)blockIdx.x + (hi ? 1 : 0);
-    // a half that has no matrix (odd batch) or that the mask skips runs on
-    // the identity and stores nothing
     bool active = b < batch;
     if (active && mask != nullptr) active = mask[b] < 0;
     const bool row_in = active && i < n;
     double *a_g = A + (active ? b : 0) * strideA;
     double *cb = CB[hi ? 1 : 0];
-    double *lt = LT[hi ? 1 : 0];
+    double *lr = LR[hi ? 1 : 0];
     double *ri = RI[hi ? 1 : 0];
+    double *sd = SD[hi ? 1 : 0];
 
     // every lane loads a valid element (indices clamped into the matrix) so
     // that the loads are unconditional and all in flight together
@@ -59,67 +72,90 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double 
         const int jc = (J < n) ? (int)J : n - 1;
         a[J] = a_g[ic + (int64_t)jc * lda];
     });
-    static_for<0, N>([&](auto J) {
-        const bool in = row_in && J < n && J <= i;
-        a[J] = in ? a[J] : (i == J ? 1.0 : 0.0);
-    });
+    {
+        const bool pad = !row_in;
+        static_for<0, N>([&](auto J) { a[J] = pad ? 0.0 : a[J]; });
+    }
 
     // factorisation; a failing half continues on a unit pivot (so no NaN
     // spreads) and reports its first failure
     int fail = 0;
-    static_for<0, N>([&](auto K) {
-        if (i < N) cb[i] = a[K];
-        // the broadcast reads below must see the other lanes' writes: lanes
-        // that wrote nothing would otherwise reuse an earlier read of cb
-        asm volatile("" ::: "memory");
-        const double d0 = cb[K];
-        const bool bad = !(d0 > 0.0);
-        if (bad && fail == 0) fail = K + 1;
-        const double d = bad ? 1.0 : d0;
-        // 1/sqrt(d): hardware estimate and one Newton step (error ~ ulp^2
-        // of the estimate, below FP64 rounding)
-        double inv = __builtin_amdgcn_rsq(d);
-        inv = inv * fma(-0.5 * d, inv * inv, 1.5);
-        const double s = d * inv;
-        ri[K] = inv;  // every lane of the half holds the same value
-        const double t = a[K] * (inv * inv);  // A[i][k] / A[k][k]
-        static_for<K + 1, N>([&](auto J) { a[J] = fma(-t, cb[J], a[J]); });
-        a[K] = (i > K) ? a[K] * inv : ((i == K) ? s : 0.0);
-    });
+    {
+        const int iw = opaque(i);
+        static_for<0, N>([&](auto K) {
+            if (i < N) cb[i] = a[K];
+            // the broadcast reads below must see the other lanes' writes: lanes
+            // that wrote nothing would otherwise reuse an earlier read of cb
+            asm volatile("" ::: "memory");
+            // the rows past n are zero, so their pivots read as failed and
+            // become 1; only failures of the real rows are reported
+            const double d0 = cb[K];
+            const bool bad = !(d0 > 0.0);
+            if (K < n && bad && fail == 0) fail = K + 1;
+            const double d = bad ? 1.0 : d0;
+            // 1/sqrt(d): hardware estimate and one Newton step (error ~ ulp^2
+            // of the estimate, below FP64 rounding)
+            double inv = __builtin_amdgcn_rsq(d);
+            inv = inv * fma(-0.5 * d, inv * inv, 1.5);
+            ri[K] = inv;  // every lane of the half holds the same values
+            sd[K] = d * inv;
+            const double t = a[K] * (inv * inv);
+            static_for<K + 1, N>([&](auto J) { a[J] = fma(-t, cb[J], a[J]); });
+            a[K] = (iw > K) ? a[K] * inv : 0.0;
+        });
+    }
     const int fail_h = hi ? __builtin_amdgcn_readlane(fail, 32) : __builtin_amdgcn_readlane(fail, 0);
     if (active && i == 0 && info != nullptr) info[b] = fail_h;
     const bool ok = active && fail_h == 0;
 
-    if (WRITE_L && ok && i < n) {
-        static_for<0, N>([&](auto J) {
-            if (J <= i && J < n) a_g[i + (int64_t)J * lda] = a[J];
+    // strictly lower L by rows for the backward solve
+    {
+        const int iw = opaque(i);
+        const int base = iw * (iw - 1) / 2;
+        static_for<0, N - 1>([&](auto J) {
+            if (i < N && iw > J) lr[base + J] = a[J];
         });
     }
-    // L by columns for the backward solve
-    static_for<0, N>([&](auto J) { lt[J * kLdT + i] = a[J]; });
+    asm volatile("" ::: "memory");
+    const double myri = ri[i < N ? i : 0];
+    if (WRITE_L && ok && i < n) {
+        const int iw = opaque(i);
+        static_for<0, N>([&](auto J) {
+            if (J < n && iw > J) a_g[i + (int64_t)J * lda] = a[J];
+        });
+        a_g[i + (int64_t)i * lda] = sd[i];
+    }
 
     const double *bb = B + (active ? b : 0) * strideB;
     double *xx = X + (active ? b : 0) * strideX;
     for (int r = 0; r < nrhs; ++r) {
         // keeps the LDS reads below inside the loop: hoisted out of it, the
-        // loop-invariant column and diagonal reads would occupy ~128 VGPRs
+        // loop-invariant reads of L would occupy ~128 VGPRs
         asm volatile("" ::: "memory");
         double y = row_in ? bb[i + (int64_t)r * ldb] : 0.0;
-        // forward: L y = b (row layout, y_k broadcast through the column buffer)
+        // forward: L y = b.  Each lane keeps its unscaled value; lane k's
+        // scaled value y_k = y * (1 / L(k, k)) is published at step k.
         static_for<0, N>([&](auto K) {
-            if (i < N) cb[i] = y;  // lanes past N own no row of the buffer
+            if (i < N) cb[i] = y * myri;
             asm volatile("" ::: "memory");
-            const double yk = cb[K] * ri[K];
-            y = (i == K) ? yk : ((i > K) ? fma(-a[K], yk, y) : y);
+            y = fma(-a[K], cb[K], y);
         });
-        // backward: L^T x = y; lane i reads column i of L from LDS
-        static_for<0, N>([&](auto tt) {
-            constexpr int K = N - 1 - tt;
-            if (i < N) cb[i] = y;  // lanes past N own no row of the buffer
-            asm volatile("" ::: "memory");
-            const double xk = cb[K] * ri[K];
-            y = (i == K) ? xk : ((i < K) ? fma(-lt[i * kLdT + K], xk, y) : y);  // L[K][i]
-        });
+        y *= myri;
+        // backward: L^T x = y; lane i reads L(k, i) from row k of the packed copy
+        {
+            const int iw = opaque(i);
+            static_for<0, N>([&](auto tt) {
+                constexpr int K = N - 1 - tt;
+                if constexpr (K > 0) {
+                    if (i < N) cb[i] = y * myri;
+                    asm volatile("" ::: "memory");
+                    constexpr int rowk = K * (K - 1) / 2;
+                    const double l = lr[rowk + (iw < K ? iw : 0)];
+                    y = fma(-(iw < K ? l : 0.0), cb[K], y);
+                }
+            });
+        }
+        y *= myri;
         if (ok && i < n) xx[i + (int64_t)r * ldx] = y;
     }
 }
