@@ -65,12 +65,16 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double 
     double *sd = SD[hi ? 1 : 0];
 
     // every lane loads a valid element (indices clamped into the matrix) so
-    // that the loads are unconditional and all in flight together
+    // that the loads are unconditional and all in flight together.  Entries
+    // above the diagonal are never used: those lanes load the diagonal
+    // element instead, so a column touches only the cache lines of its lower
+    // part.
     const int ic = (i < n) ? i : n - 1;
     double a[N];
     static_for<0, N>([&](auto J) {
         const int jc = (J < n) ? (int)J : n - 1;
-        a[J] = a_g[ic + (int64_t)jc * lda];
+        const int r = ic > jc ? ic : jc;
+        a[J] = a_g[r + (int64_t)jc * lda];
     });
     {
         const bool pad = !row_in;
