@@ -42,6 +42,10 @@ __device__ __forceinline__ int opaque(int v) {
     return v;
 }
 
+// the value is materialised here: the compiler may not defer the arithmetic
+// producing it past this point
+__device__ __forceinline__ void pin(double &x) { asm volatile("" : "+v"(x)); }
+
 template <int N, bool WRITE_L>
 __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double *A, int64_t lda, int64_t strideA,
                                                         const double *B, int64_t ldb, int64_t strideB, double *X,
@@ -106,6 +110,10 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double 
             const double t = a[K] * (inv * inv);
             static_for<K + 1, N>([&](auto J) { a[J] = fma(-t, cb[J], a[J]); });
             a[K] = (iw > K) ? a[K] * inv : 0.0;
+            // finish this step's row update here: deferred to the step that
+            // first needs each element, it keeps every earlier broadcast
+            // column live (over 200 VGPRs instead of 127)
+            static_for<K, N>([&](auto J) { pin(a[J]); });
         });
     }
     const int fail_h = hi ? __builtin_amdgcn_readlane(fail, 32) : __builtin_amdgcn_readlane(fail, 0);
