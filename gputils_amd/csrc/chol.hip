@@ -101,6 +101,9 @@ __device__ __forceinline__ void tri_inverse_cols(const T (&row)[16], const T (&i
 #pragma unroll
         for (int p = 0; p < r; ++p) acc -= readlane(row[p], r) * x[p];
         x[r] = acc * invd[r];
+        // x[r] is computed here: left to the compiler, every row's readlane
+        // results were hoisted into SGPRs and spilled (about 450 spills)
+        asm volatile("" : "+v"(x[r]));
     }
 }
 

@@ -266,3 +266,58 @@ def test_small_fused_posv_failure_and_upper_untouched(oracle, n):
     X = tB.cpu().numpy()
     for b in (0, 2, 3):
         assert backward_error(A[b:b + 1], X[b:b + 1], B[b:b + 1]) <= 1e-14
+
+
+@pytest.mark.parametrize("n", [7, 16, 29, 32])
+def test_small_fused_posv_ignores_upper_triangle(n):
+    """n <= 32 fused path: only the lower triangle is an input.  NaN and inf
+    in the strictly upper triangle give the same L and X, bit for bit, as a
+    zero upper triangle (potrf's "upper part not referenced")."""
+    k = 5
+    A = spd_batch(k, n, seed=300 + n)
+    B = np.random.default_rng(n).standard_normal((k, 2, n))
+    results = []
+    upper = np.triu(np.ones((n, n), dtype=bool), 1)
+    for fill in (0.0, np.nan, np.inf):
+        M = np.array([np.where(upper, fill, np.tril(A[b])) for b in range(k)])
+        tA = dev(api.from_matrices(torch.from_numpy(M)).numpy())
+        tB = dev(B)
+        info = api.posv_batched(tA, tB).cpu().tolist()
+        assert info == [0] * k
+        L = np.tril(api.to_matrices(tA).cpu().numpy())
+        results.append((L, tB.cpu().numpy()))
+    for L, X in results[1:]:
+        assert np.array_equal(L, results[0][0]) and np.array_equal(X, results[0][1])
+    assert backward_error(A, results[0][1], B) <= 1e-14
+
+
+@pytest.mark.parametrize("n,nrhs", [(9, 2), (32, 3)])
+def test_small_fused_posv_padded_leading_dimensions(oracle, n, nrhs):
+    """The C entry point with lda = n + 3, ldb = n + 1 and batch strides
+    larger than one matrix: same factor and solution as the oracle, and the
+    padding rows and the upper triangle are left as they were."""
+    k, lda, ldb = 5, n + 3, n + 1
+    A = spd_batch(k, n, seed=400 + n)
+    B = np.random.default_rng(7).standard_normal((k, nrhs, n))
+    Ab = np.full((k, n + 1, lda), 5.25)   # column j of matrix b at Ab[b, j, :]; one spare column per matrix
+    for b in range(k):
+        Ab[b, :n, :n] = A[b].T
+    Bb = np.full((k, nrhs + 1, ldb), -3.5)
+    Bb[:, :nrhs, :n] = B
+    tA, tB = dev(Ab), dev(Bb)
+    info = torch.empty(k, dtype=torch.int32, device="cuda")
+    rc = api.lib().gputils_dposv_batched(n, nrhs, api._ptr(tA), lda, (n + 1) * lda, api._ptr(tB), ldb,
+                                         (nrhs + 1) * ldb, api._ptr(info), k, api.stream_handle())
+    assert rc == 0
+    torch.cuda.synchronize()
+    assert info.cpu().tolist() == [0] * k
+    Ao, Bo = tA.cpu().numpy(), tB.cpu().numpy()
+    for b in range(k):
+        Lg = Ao[b, :n, :n].T                                  # row i, column j
+        Lo, _ = oracle.potrf(A[b])
+        assert np.max(np.abs(np.tril(Lg) - np.tril(Lo))) <= 1e-12 * np.abs(Lo).max()
+        assert np.array_equal(np.triu(Lg, 1), np.triu(A[b], 1))
+        ref = oracle.potrs(Lo, B[b].T)
+        assert np.max(np.abs(Bo[b, :nrhs, :n].T - ref)) <= 1e-11 * np.abs(ref).max()
+    assert np.all(Ao[:, :n, n:] == 5.25) and np.all(Ao[:, n, :] == 5.25)
+    assert np.all(Bo[:, :nrhs, n:] == -3.5) and np.all(Bo[:, nrhs, :] == -3.5)
