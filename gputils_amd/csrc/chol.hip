@@ -113,7 +113,11 @@ __device__ __forceinline__ void tri_inverse_cols(const T (&row)[16], const T (&i
 // Returns the index (0..15) of the first non-positive pivot or -1.
 template <typename T>
 __device__ __forceinline__ int factor_diag_block(T *blk, T *Lout, int64_t ldo, int nvalid) {
-    const int lane = threadIdx.x & 63;
+    // a lane index the compiler cannot relate to the one of the other
+    // phases, so the per-step lane masks below are formed where they are
+    // used instead of being kept live in SGPRs across the kernel
+    int lane = threadIdx.x & 63;
+    asm volatile("" : "+v"(lane));
     T row[16], invd[16];
 #pragma unroll
     for (int c = 0; c < 16; ++c) row[c] = blk[(lane & 15) + 16 * c];
