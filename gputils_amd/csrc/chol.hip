@@ -401,6 +401,15 @@ __global__ __launch_bounds__(kThreads) void posv_kernel(int n, int nrhs, T *A, i
     __shared__ T L[tri(NB) * 256];
     __shared__ T Y[16 * NB * kRhsChunk];
     __shared__ int s_flag;
+    if (mask != nullptr) {
+        // a masked launch mostly finds nothing to do: read this workgroup's
+        // share of the mask at once (one entry per thread) instead of one
+        // dependent load per matrix, and leave if none is selected
+        const int64_t bt = blockIdx.x + (int64_t)threadIdx.x * gridDim.x;
+        const int need = (bt < batch && mask[bt] < 0) ? 1 : 0;
+        const bool covered = (int64_t)kThreads * gridDim.x >= batch;  // uniform
+        if (!__syncthreads_or(need) && covered) return;
+    }
     for (int64_t b = blockIdx.x; b < batch; b += gridDim.x) {
         if (mask != nullptr && mask[b] >= 0) continue;
         const int nb = (n + 15) >> 4;

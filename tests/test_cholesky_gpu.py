@@ -321,3 +321,31 @@ def test_small_fused_posv_padded_leading_dimensions(oracle, n, nrhs):
         assert np.max(np.abs(Bo[b, :nrhs, :n].T - ref)) <= 1e-11 * np.abs(ref).max()
     assert np.all(Ao[:, :n, n:] == 5.25) and np.all(Ao[:, n, :] == 5.25)
     assert np.all(Bo[:, :nrhs, n:] == -3.5) and np.all(Bo[:, nrhs, :] == -3.5)
+
+
+def test_mixed_fallback_many_systems_per_workgroup():
+    """The FP64 fallback launch covers the batch with 512 workgroups, so at
+    k = 1100 a workgroup owns up to three systems (b, b + 512, b + 1024).
+    Systems 5, 517 and 1029 share one workgroup: an indefinite matrix, an
+    SPD one and an ill-conditioned one (the FP32 factor is unusable, FP64
+    solves it).  Every other system converges in mixed precision."""
+    k, n = 1100, 40
+    r = np.random.default_rng(11)
+    A = spd_batch(k, n, seed=12)
+    Q, _ = np.linalg.qr(r.standard_normal((n, n)))
+    ill = (Q * np.logspace(0, -12, n)) @ Q.T
+    A[1029] = 0.5 * (ill + ill.T)
+    A[5] = -A[5]
+    A[600] = -A[600]
+    B = r.standard_normal((k, 2, n))
+    X, iters, info = api.posv_mixed_batched(dev(A), dev(B), tol=1e-12, max_iter=10)
+    it, inf = iters.cpu().numpy(), info.cpu().numpy()
+    X = X.cpu().numpy()
+    assert it[5] == -1 and inf[5] == 1
+    assert it[600] == -1 and inf[600] == 1
+    assert it[1029] == -1 and inf[1029] == 0
+    ok = np.ones(k, dtype=bool)
+    ok[[5, 600, 1029]] = False
+    assert np.all(it[ok] >= 0) and np.all(inf[ok] == 0)
+    assert backward_error(A[ok], X[ok], B[ok]) <= 1e-12
+    assert backward_error(A[1029:1030], X[1029:1030], B[1029:1030]) <= 1e-13
