@@ -349,3 +349,37 @@ def test_mixed_fallback_many_systems_per_workgroup():
     assert np.all(it[ok] >= 0) and np.all(inf[ok] == 0)
     assert backward_error(A[ok], X[ok], B[ok]) <= 1e-12
     assert backward_error(A[1029:1030], X[1029:1030], B[1029:1030]) <= 1e-13
+
+
+@pytest.mark.parametrize("n", [48, 128])
+def test_blocked_and_mixed_ignore_upper_triangle(n):
+    """n > 32 (LDS-blocked FP64 kernel) and the mixed solver: NaN or inf in
+    the strictly upper triangle of A gives the same L, X, iterations and
+    info, bit for bit, as a zero upper triangle, and the upper triangle is
+    not written."""
+    k = 6
+    A = spd_batch(k, n, seed=500 + n)
+    B = np.random.default_rng(n + 1).standard_normal((k, 3, n))
+    upper = np.triu(np.ones((n, n), dtype=bool), 1)
+    fused, mixed = [], []
+    for fill in (0.0, np.nan, np.inf):
+        M = np.array([np.where(upper, fill, np.tril(A[b])) for b in range(k)])
+        Mt = api.from_matrices(torch.from_numpy(M)).numpy()
+        tA, tB = dev(Mt), dev(B)
+        X, it, inf = api.posv_mixed_batched(tA, tB, tol=1e-12, max_iter=10)
+        mixed.append((X.cpu().numpy(), it.cpu().numpy(), inf.cpu().numpy()))
+        info = api.posv_batched(tA, tB).cpu().tolist()
+        assert info == [0] * k
+        L = api.to_matrices(tA).cpu().numpy()
+        if fill == 0.0:
+            assert np.all(L[:, upper] == 0.0)
+        elif np.isnan(fill):
+            assert np.all(np.isnan(L[:, upper]))
+        else:
+            assert np.all(L[:, upper] == fill)
+        fused.append((np.tril(L), tB.cpu().numpy()))
+    for (L, X), (Xm, itm, infm) in zip(fused[1:], mixed[1:]):
+        assert np.array_equal(L, fused[0][0]) and np.array_equal(X, fused[0][1])
+        assert np.array_equal(Xm, mixed[0][0]) and np.array_equal(itm, mixed[0][1]) and np.array_equal(infm, mixed[0][2])
+    assert backward_error(A, fused[0][1], B) <= 1e-14
+    assert backward_error(A, mixed[0][0], B) <= 1e-12
