@@ -1,11 +1,12 @@
 // chol_small.hip -- FP64 batched Cholesky factor + solve for n <= 32 on
-// gfx950: two matrices per wavefront (one per 32-lane half), lane i of a
-// half owns row i of its matrix in registers.
+// gfx950: N = 32 (n > 16) puts two matrices in a wavefront (one per 32-lane
+// group), N = 16 four (one per 16-lane group); lane i of a group owns row i
+// of its matrix in registers.
 //
 // Used for the small-n cases of cusolverDn?potrfBatched / ?potrsBatched
 // behind CholeskyBatchFactoriser (include/tensor.cuh:2135-2197) when the
 // factor and the solve are fused.  Right-looking unblocked algorithm: at
-// step k every lane writes its current A[i][k] to a per-half LDS column
+// step k every lane writes its current A[i][k] to a per-group LDS column
 // buffer and reads the pivot and the column back (broadcast reads), then
 // updates its row with A[i][j] -= A[i][k] A[j][k] / A[k][k].  Only the lower
 // triangle is used: the entries right of the diagonal, loaded with the row,
@@ -52,21 +53,21 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double 
                                                         int64_t ldx, int64_t strideX, int *info, int64_t batch,
                                                         const int *mask) {
     constexpr int kTri = N * (N - 1) / 2;
-    __shared__ __attribute__((aligned(16))) double LR[2][kTri];  // strict lower L, packed by rows
-    __shared__ __attribute__((aligned(16))) double CB[2][N];     // per-half broadcast buffer
-    __shared__ double RI[2][N];                                   // per-half 1 / L(k, k)
-    __shared__ double SD[2][N];                                   // per-half L(k, k)
-    const int lane = threadIdx.x, i = lane & 31;
-    const bool hi = lane >= 32;
-    const int64_t b = 2 * (int64_t)blockIdx.x + (hi ? 1 : 0);
+    constexpr int G = 64 / N;  // matrices per wavefront, one per group of N lanes
+    __shared__ __attribute__((aligned(16))) double LR[G][kTri];  // strict lower L, packed by rows
+    __shared__ __attribute__((aligned(16))) double CB[G][N];     // per-group broadcast buffer
+    __shared__ double RI[G][N];                                   // per-group 1 / L(k, k)
+    __shared__ double SD[G][N];                                   // per-group L(k, k)
+    const int lane = threadIdx.x, i = lane % N, grp = lane / N;
+    const int64_t b = G * (int64_t)blockIdx.x + grp;
     bool active = b < batch;
     if (active && mask != nullptr) active = mask[b] < 0;
     const bool row_in = active && i < n;
     double *a_g = A + (active ? b : 0) * strideA;
-    double *cb = CB[hi ? 1 : 0];
-    double *lr = LR[hi ? 1 : 0];
-    double *ri = RI[hi ? 1 : 0];
-    double *sd = SD[hi ? 1 : 0];
+    double *cb = CB[grp];
+    double *lr = LR[grp];
+    double *ri = RI[grp];
+    double *sd = SD[grp];
 
     // every lane loads a valid element (indices clamped into the matrix) so
     // that the loads are unconditional and all in flight together.  Entries
@@ -85,7 +86,7 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double 
         static_for<0, N>([&](auto J) { a[J] = pad ? 0.0 : a[J]; });
     }
 
-    // factorisation; a failing half continues on a unit pivot (so no NaN
+    // factorisation; a failing group continues on a unit pivot (so no NaN
     // spreads) and reports its first failure
     int fail = 0;
     {
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double 
             // of the estimate, below FP64 rounding)
             double inv = __builtin_amdgcn_rsq(d);
             inv = inv * fma(-0.5 * d, inv * inv, 1.5);
-            ri[K] = inv;  // every lane of the half holds the same values
+            ri[K] = inv;  // every lane of the group holds the same values
             sd[K] = d * inv;
             const double t = a[K] * (inv * inv);
             static_for<K + 1, N>([&](auto J) { a[J] = fma(-t, cb[J], a[J]); });
@@ -116,7 +117,14 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double 
             static_for<K, N>([&](auto J) { pin(a[J]); });
         });
     }
-    const int fail_h = hi ? __builtin_amdgcn_readlane(fail, 32) : __builtin_amdgcn_readlane(fail, 0);
+    // every lane of a group read the same pivots, so fail is the group's;
+    // taking it from the group's first lane makes that explicit
+    int fail_h = __builtin_amdgcn_readlane(fail, 0);
+    static_for<1, G>([&](auto Gc) {
+        constexpr int g = Gc;
+        const int f = __builtin_amdgcn_readlane(fail, g * N);
+        fail_h = (grp == g) ? f : fail_h;
+    });
     if (active && i == 0 && info != nullptr) info[b] = fail_h;
     const bool ok = active && fail_h == 0;
 
@@ -175,7 +183,8 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double 
 int launch_posv_small(int n, int nrhs, double *A, int64_t lda, int64_t strideA, const double *B, int64_t ldb,
                       int64_t strideB, double *X, int64_t ldx, int64_t strideX, int *info, int64_t batch,
                       const int *mask, bool write_l, hipStream_t st) {
-    const int64_t blocks = (batch + 1) / 2;
+    const int64_t per_wave = (n <= 16) ? 4 : 2;
+    const int64_t blocks = (batch + per_wave - 1) / per_wave;
     const dim3 grid((unsigned)blocks), block(64);
 #define GPUTILS_SMALL(NV, WL)                                                                                        \
     hipLaunchKernelGGL((posv_small_kernel<NV, WL>), grid, block, 0, st, n, nrhs, A, lda, strideA, B, ldb, strideB, X, \

@@ -9,7 +9,7 @@
 namespace gputils {
 namespace chols {
 
-// Two matrices per wavefront.  write_l: A_i <- L_i (lower triangle).
+// Two matrices per wavefront (four for n <= 16).  write_l: A_i <- L_i (lower triangle).
 // mask != nullptr: only matrices with mask[b] < 0 are processed.
 int launch_posv_small(int n, int nrhs, double *A, int64_t lda, int64_t strideA, const double *B, int64_t ldb,
                       int64_t strideB, double *X, int64_t ldx, int64_t strideX, int *info, int64_t batch,
