@@ -513,6 +513,13 @@ __global__ __launch_bounds__(kThreads) void potrs_global_kernel(int n, int nrhs,
 template <typename T>
 int potrf_dispatch(int64_t n, T *A, int64_t lda, int64_t strideA, int *info, int64_t batch,
                    hipStream_t st) {
+    if constexpr (std::is_same_v<T, double>) {
+        // factor only: the register-resident small kernel with no right-hand
+        // sides (B and X are not read or written when nrhs = 0)
+        if (n <= 32)
+            return chols::launch_posv_small((int)n, 0, A, lda, strideA, A, lda, strideA, A, lda, strideA, info, batch,
+                                            nullptr, true, st);
+    }
     const dim3 grid((unsigned)batch), block(kThreads);
     if (n <= 16) hipLaunchKernelGGL((potrf_kernel<T, 1>), grid, block, 0, st, (int)n, A, lda, strideA, info, batch);
     else if (n <= 32) hipLaunchKernelGGL((potrf_kernel<T, 2>), grid, block, 0, st, (int)n, A, lda, strideA, info, batch);
