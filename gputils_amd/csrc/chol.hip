@@ -532,6 +532,10 @@ int potrf_dispatch(int64_t n, T *A, int64_t lda, int64_t strideA, int *info, int
 template <typename T>
 int potrs_dispatch(int64_t n, int64_t nrhs, const T *L, int64_t lda, int64_t strideL, T *B,
                    int64_t ldb, int64_t strideB, int *info, int64_t batch, hipStream_t st) {
+    if constexpr (std::is_same_v<T, double>) {
+        if (n <= 32)
+            return chols::launch_potrs_small((int)n, (int)nrhs, L, lda, strideL, B, ldb, strideB, info, batch, st);
+    }
     const dim3 grid((unsigned)batch), block(kThreads);
 #define GPUTILS_POTRS(NBV) \
     hipLaunchKernelGGL((potrs_kernel<T, NBV>), grid, block, 0, st, (int)n, (int)nrhs, L, lda, strideL, B, ldb, strideB, info, batch)

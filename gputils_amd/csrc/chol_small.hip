@@ -47,7 +47,7 @@ __device__ __forceinline__ int opaque(int v) {
 // producing it past this point
 __device__ __forceinline__ void pin(double &x) { asm volatile("" : "+v"(x)); }
 
-template <int N, bool WRITE_L>
+template <int N, bool WRITE_L, bool GIVEN_L>
 __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double *A, int64_t lda, int64_t strideA,
                                                         const double *B, int64_t ldb, int64_t strideB, double *X,
                                                         int64_t ldx, int64_t strideX, int *info, int64_t batch,
@@ -89,7 +89,18 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double 
     // factorisation; a failing group continues on a unit pivot (so no NaN
     // spreads) and reports its first failure
     int fail = 0;
-    {
+    double myri_given = 1.0;
+    if constexpr (GIVEN_L) {
+        // the factor is an input (solve only): keep its strictly lower part
+        // and 1 / L(i, i); rows past n get a unit diagonal
+        const int iw = opaque(i);
+        double diag = 1.0;
+        static_for<0, N>([&](auto J) {
+            diag = (iw == J) ? a[J] : diag;
+            a[J] = (iw > J) ? a[J] : 0.0;
+        });
+        myri_given = 1.0 / (row_in ? diag : 1.0);
+    } else {
         const int iw = opaque(i);
         static_for<0, N>([&](auto K) {
             if (i < N) cb[i] = a[K];
@@ -137,7 +148,9 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double 
         });
     }
     asm volatile("" ::: "memory");
-    const double myri = ri[i < N ? i : 0];
+    double myri;
+    if constexpr (GIVEN_L) myri = myri_given;
+    else myri = ri[i < N ? i : 0];
     if (WRITE_L && ok && i < n) {
         const int iw = opaque(i);
         static_for<0, N>([&](auto J) {
@@ -187,7 +200,7 @@ int launch_posv_small(int n, int nrhs, double *A, int64_t lda, int64_t strideA, 
     const int64_t blocks = (batch + per_wave - 1) / per_wave;
     const dim3 grid((unsigned)blocks), block(64);
 #define GPUTILS_SMALL(NV, WL)                                                                                        \
-    hipLaunchKernelGGL((posv_small_kernel<NV, WL>), grid, block, 0, st, n, nrhs, A, lda, strideA, B, ldb, strideB, X, \
+    hipLaunchKernelGGL((posv_small_kernel<NV, WL, false>), grid, block, 0, st, n, nrhs, A, lda, strideA, B, ldb, strideB, X, \
                        ldx, strideX, info, batch, mask)
     if (n <= 16) {
         if (write_l) GPUTILS_SMALL(16, true);
@@ -198,6 +211,22 @@ int launch_posv_small(int n, int nrhs, double *A, int64_t lda, int64_t strideA, 
     }
 #undef GPUTILS_SMALL
     return launch_check("posv_small_kernel");
+}
+
+int launch_potrs_small(int n, int nrhs, const double *L, int64_t lda, int64_t strideL, double *B, int64_t ldb,
+                       int64_t strideB, int *info, int64_t batch, hipStream_t st) {
+    const int64_t per_wave = (n <= 16) ? 4 : 2;
+    const int64_t blocks = (batch + per_wave - 1) / per_wave;
+    const dim3 grid((unsigned)blocks), block(64);
+    // L is only read (WRITE_L = false); X aliases B
+    double *l = const_cast<double *>(L);
+    if (n <= 16)
+        hipLaunchKernelGGL((posv_small_kernel<16, false, true>), grid, block, 0, st, n, nrhs, l, lda, strideL, B, ldb,
+                           strideB, B, ldb, strideB, info, batch, nullptr);
+    else
+        hipLaunchKernelGGL((posv_small_kernel<32, false, true>), grid, block, 0, st, n, nrhs, l, lda, strideL, B, ldb,
+                           strideB, B, ldb, strideB, info, batch, nullptr);
+    return launch_check("posv_small_kernel (solve only)");
 }
 
 }  // namespace chols
