@@ -19,7 +19,8 @@
 // (value * 1 / L(k, k)) at step k, which is the same product the scaled
 // form computes; the backward solve reads L(k, i) from a copy of the
 // strictly lower L packed by rows in LDS (4 KB per matrix for n = 32, so a
-// CU's LDS holds sixteen workgroups).
+// CU's LDS holds sixteen workgroups).  The same kernel factors only
+// (nrhs = 0, potrf) and solves only (GIVEN_L: L is read, potrs).
 #include "chol_small.hpp"
 #include "common.hpp"
 

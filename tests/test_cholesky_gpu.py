@@ -383,3 +383,32 @@ def test_blocked_and_mixed_ignore_upper_triangle(n):
         assert np.array_equal(Xm, mixed[0][0]) and np.array_equal(itm, mixed[0][1]) and np.array_equal(infm, mixed[0][2])
     assert backward_error(A, fused[0][1], B) <= 1e-14
     assert backward_error(A, mixed[0][0], B) <= 1e-12
+
+
+@pytest.mark.parametrize("n,nrhs", [(9, 3), (16, 2), (32, 3)])
+def test_small_potrs_given_factor_padded(oracle, n, nrhs):
+    """n <= 32 solve only (potrs) from the oracle's factor, through the C
+    entry point with lda = n + 2, ldb = n + 3: matches the oracle's solve,
+    leaves L (both triangles) and the padding untouched."""
+    k, lda, ldb = 7, n + 2, n + 3
+    A = spd_batch(k, n, seed=600 + n)
+    Ls = [oracle.potrf(A[b])[0] for b in range(k)]
+    B = np.random.default_rng(9).standard_normal((k, nrhs, n))
+    Lb = np.full((k, n, lda), 2.5)
+    for b in range(k):
+        Lb[b, :, :n] = (np.tril(Ls[b]) + np.triu(np.full((n, n), 4.0), 1)).T   # column j at Lb[b, j, :]
+    Bb = np.full((k, nrhs, ldb), -1.5)
+    Bb[:, :, :n] = B
+    tL, tB = dev(Lb), dev(Bb)
+    info = torch.full((k,), 99, dtype=torch.int32, device="cuda")
+    rc = api.lib().gputils_dpotrs_batched(n, nrhs, api._ptr(tL), lda, n * lda, api._ptr(tB), ldb, nrhs * ldb,
+                                          api._ptr(info), k, api.stream_handle())
+    assert rc == 0
+    torch.cuda.synchronize()
+    assert info.cpu().tolist() == [0] * k
+    assert np.array_equal(tL.cpu().numpy(), Lb)
+    Xo = tB.cpu().numpy()
+    assert np.all(Xo[:, :, n:] == -1.5)
+    for b in range(k):
+        ref = oracle.potrs(Ls[b], B[b].T)
+        assert np.max(np.abs(Xo[b, :, :n].T - ref)) <= 1e-11 * np.abs(ref).max()
