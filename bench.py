@@ -374,7 +374,7 @@ def bench_cholesky_small(a):
                  {"workload": "cholesky_small: k=131072 (and 1024) SPD 32x32, A = M M^T + 32 I, 1 RHS",
                   "k": 131072, "n": 32},
                  _roofline("hbm", 131072 * (2 * (32 * 33 // 2) * 8 + 2 * 32 * 8), big["event_ms"],
-                           "posv_small_kernel<32,true,false>",
+                           "posv_small_kernel<double,32,true,false>",
                            "lower triangle of A read and of L written, b read and x written (8 B each)"),
                  cpu, results=res)
 

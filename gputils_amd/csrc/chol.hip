@@ -513,13 +513,11 @@ __global__ __launch_bounds__(kThreads) void potrs_global_kernel(int n, int nrhs,
 template <typename T>
 int potrf_dispatch(int64_t n, T *A, int64_t lda, int64_t strideA, int *info, int64_t batch,
                    hipStream_t st) {
-    if constexpr (std::is_same_v<T, double>) {
-        // factor only: the register-resident small kernel with no right-hand
-        // sides (B and X are not read or written when nrhs = 0)
-        if (n <= 32)
-            return chols::launch_posv_small((int)n, 0, A, lda, strideA, A, lda, strideA, A, lda, strideA, info, batch,
-                                            nullptr, true, st);
-    }
+    // factor only: the register-resident small kernel with no right-hand
+    // sides (B and X are not read or written when nrhs = 0)
+    if (n <= 32)
+        return chols::launch_posv_small<T>((int)n, 0, A, lda, strideA, A, lda, strideA, A, lda, strideA, info, batch,
+                                           nullptr, true, st);
     const dim3 grid((unsigned)batch), block(kThreads);
     if (n <= 16) hipLaunchKernelGGL((potrf_kernel<T, 1>), grid, block, 0, st, (int)n, A, lda, strideA, info, batch);
     else if (n <= 32) hipLaunchKernelGGL((potrf_kernel<T, 2>), grid, block, 0, st, (int)n, A, lda, strideA, info, batch);
@@ -532,10 +530,8 @@ int potrf_dispatch(int64_t n, T *A, int64_t lda, int64_t strideA, int *info, int
 template <typename T>
 int potrs_dispatch(int64_t n, int64_t nrhs, const T *L, int64_t lda, int64_t strideL, T *B,
                    int64_t ldb, int64_t strideB, int *info, int64_t batch, hipStream_t st) {
-    if constexpr (std::is_same_v<T, double>) {
-        if (n <= 32)
-            return chols::launch_potrs_small((int)n, (int)nrhs, L, lda, strideL, B, ldb, strideB, info, batch, st);
-    }
+    if (n <= 32)
+        return chols::launch_potrs_small<T>((int)n, (int)nrhs, L, lda, strideL, B, ldb, strideB, info, batch, st);
     const dim3 grid((unsigned)batch), block(kThreads);
 #define GPUTILS_POTRS(NBV) \
     hipLaunchKernelGGL((potrs_kernel<T, NBV>), grid, block, 0, st, (int)n, (int)nrhs, L, lda, strideL, B, ldb, strideB, info, batch)
@@ -552,11 +548,9 @@ template <typename T, bool WRITE_L>
 int posv_dispatch(int64_t n, int64_t nrhs, T *A, int64_t lda, int64_t strideA, const T *B,
                   int64_t ldb, int64_t strideB, T *X, int64_t ldx, int64_t strideX, int *info,
                   int64_t batch, const int *mask, hipStream_t st) {
-    if constexpr (std::is_same_v<T, double>) {
-        if (n <= 32)
-            return chols::launch_posv_small((int)n, (int)nrhs, A, lda, strideA, B, ldb, strideB, X, ldx, strideX, info,
-                                            batch, mask, WRITE_L, st);
-    }
+    if (n <= 32)
+        return chols::launch_posv_small<T>((int)n, (int)nrhs, A, lda, strideA, B, ldb, strideB, X, ldx, strideX, info,
+                                           batch, mask, WRITE_L, st);
     // masked (fallback) launches scan the mask with a grid-stride loop: two
     // workgroups per CU fill the GPU at this kernel's LDS size
     const int64_t g = (mask != nullptr) ? (batch < 512 ? batch : 512) : batch;

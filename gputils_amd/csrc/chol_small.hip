@@ -1,4 +1,4 @@
-// chol_small.hip -- FP64 batched Cholesky factor + solve for n <= 32 on
+// chol_small.hip -- FP32/FP64 batched Cholesky factor + solve for n <= 32 on
 // gfx950: N = 32 (n > 16) puts two matrices in a wavefront (one per 32-lane
 // group), N = 16 four (one per 16-lane group); lane i of a group owns row i
 // of its matrix in registers.
@@ -46,29 +46,34 @@ __device__ __forceinline__ int opaque(int v) {
 
 // the value is materialised here: the compiler may not defer the arithmetic
 // producing it past this point
-__device__ __forceinline__ void pin(double &x) { asm volatile("" : "+v"(x)); }
+template <typename T>
+__device__ __forceinline__ void pin(T &x) { asm volatile("" : "+v"(x)); }
 
-template <int N, bool WRITE_L, bool GIVEN_L>
-__global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double *A, int64_t lda, int64_t strideA,
-                                                        const double *B, int64_t ldb, int64_t strideB, double *X,
+// hardware reciprocal square root estimate (refined by the caller)
+__device__ __forceinline__ double rsq_estimate(double d) { return __builtin_amdgcn_rsq(d); }
+__device__ __forceinline__ float rsq_estimate(float d) { return __builtin_amdgcn_rsqf(d); }
+
+template <typename T, int N, bool WRITE_L, bool GIVEN_L>
+__global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, T *A, int64_t lda, int64_t strideA,
+                                                        const T *B, int64_t ldb, int64_t strideB, T *X,
                                                         int64_t ldx, int64_t strideX, int *info, int64_t batch,
                                                         const int *mask) {
     constexpr int kTri = N * (N - 1) / 2;
     constexpr int G = 64 / N;  // matrices per wavefront, one per group of N lanes
-    __shared__ __attribute__((aligned(16))) double LR[G][kTri];  // strict lower L, packed by rows
-    __shared__ __attribute__((aligned(16))) double CB[G][N];     // per-group broadcast buffer
-    __shared__ double RI[G][N];                                   // per-group 1 / L(k, k)
-    __shared__ double SD[G][N];                                   // per-group L(k, k)
+    __shared__ __attribute__((aligned(16))) T LR[G][kTri];  // strict lower L, packed by rows
+    __shared__ __attribute__((aligned(16))) T CB[G][N];     // per-group broadcast buffer
+    __shared__ T RI[G][N];                                   // per-group 1 / L(k, k)
+    __shared__ T SD[G][N];                                   // per-group L(k, k)
     const int lane = threadIdx.x, i = lane % N, grp = lane / N;
     const int64_t b = G * (int64_t)blockIdx.x + grp;
     bool active = b < batch;
     if (active && mask != nullptr) active = mask[b] < 0;
     const bool row_in = active && i < n;
-    double *a_g = A + (active ? b : 0) * strideA;
-    double *cb = CB[grp];
-    double *lr = LR[grp];
-    double *ri = RI[grp];
-    double *sd = SD[grp];
+    T *a_g = A + (active ? b : 0) * strideA;
+    T *cb = CB[grp];
+    T *lr = LR[grp];
+    T *ri = RI[grp];
+    T *sd = SD[grp];
 
     // every lane loads a valid element (indices clamped into the matrix) so
     // that the loads are unconditional and all in flight together.  Entries
@@ -76,7 +81,7 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double 
     // element instead, so a column touches only the cache lines of its lower
     // part.
     const int ic = (i < n) ? i : n - 1;
-    double a[N];
+    T a[N];
     static_for<0, N>([&](auto J) {
         const int jc = (J < n) ? (int)J : n - 1;
         const int r = ic > jc ? ic : jc;
@@ -84,23 +89,23 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double 
     });
     {
         const bool pad = !row_in;
-        static_for<0, N>([&](auto J) { a[J] = pad ? 0.0 : a[J]; });
+        static_for<0, N>([&](auto J) { a[J] = pad ? T(0) : a[J]; });
     }
 
     // factorisation; a failing group continues on a unit pivot (so no NaN
     // spreads) and reports its first failure
     int fail = 0;
-    double myri_given = 1.0;
+    T myri_given = T(1);
     if constexpr (GIVEN_L) {
         // the factor is an input (solve only): keep its strictly lower part
         // and 1 / L(i, i); rows past n get a unit diagonal
         const int iw = opaque(i);
-        double diag = 1.0;
+        T diag = T(1);
         static_for<0, N>([&](auto J) {
             diag = (iw == J) ? a[J] : diag;
-            a[J] = (iw > J) ? a[J] : 0.0;
+            a[J] = (iw > J) ? a[J] : T(0);
         });
-        myri_given = 1.0 / (row_in ? diag : 1.0);
+        myri_given = T(1) / (row_in ? diag : T(1));
     } else {
         const int iw = opaque(i);
         static_for<0, N>([&](auto K) {
@@ -110,19 +115,19 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double 
             asm volatile("" ::: "memory");
             // the rows past n are zero, so their pivots read as failed and
             // become 1; only failures of the real rows are reported
-            const double d0 = cb[K];
-            const bool bad = !(d0 > 0.0);
+            const T d0 = cb[K];
+            const bool bad = !(d0 > T(0));
             if (K < n && bad && fail == 0) fail = K + 1;
-            const double d = bad ? 1.0 : d0;
-            // 1/sqrt(d): hardware estimate and one Newton step (error ~ ulp^2
-            // of the estimate, below FP64 rounding)
-            double inv = __builtin_amdgcn_rsq(d);
-            inv = inv * fma(-0.5 * d, inv * inv, 1.5);
+            const T d = bad ? T(1) : d0;
+            // 1/sqrt(d): hardware estimate and one Newton step in T (error ~
+            // ulp^2 of the estimate, below T's rounding)
+            T inv = rsq_estimate(d);
+            inv = inv * fma(T(-0.5) * d, inv * inv, T(1.5));
             ri[K] = inv;  // every lane of the group holds the same values
             sd[K] = d * inv;
-            const double t = a[K] * (inv * inv);
+            const T t = a[K] * (inv * inv);
             static_for<K + 1, N>([&](auto J) { a[J] = fma(-t, cb[J], a[J]); });
-            a[K] = (iw > K) ? a[K] * inv : 0.0;
+            a[K] = (iw > K) ? a[K] * inv : T(0);
             // finish this step's row update here: deferred to the step that
             // first needs each element, it keeps every earlier broadcast
             // column live (over 200 VGPRs instead of 127)
@@ -149,7 +154,7 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double 
         });
     }
     asm volatile("" ::: "memory");
-    double myri;
+    T myri;
     if constexpr (GIVEN_L) myri = myri_given;
     else myri = ri[i < N ? i : 0];
     if (WRITE_L && ok && i < n) {
@@ -160,13 +165,13 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double 
         a_g[i + (int64_t)i * lda] = sd[i];
     }
 
-    const double *bb = B + (active ? b : 0) * strideB;
-    double *xx = X + (active ? b : 0) * strideX;
+    const T *bb = B + (active ? b : 0) * strideB;
+    T *xx = X + (active ? b : 0) * strideX;
     for (int r = 0; r < nrhs; ++r) {
         // keeps the LDS reads below inside the loop: hoisted out of it, the
         // loop-invariant reads of L would occupy ~128 VGPRs
         asm volatile("" ::: "memory");
-        double y = row_in ? bb[i + (int64_t)r * ldb] : 0.0;
+        T y = row_in ? bb[i + (int64_t)r * ldb] : T(0);
         // forward: L y = b.  Each lane keeps its unscaled value; lane k's
         // scaled value y_k = y * (1 / L(k, k)) is published at step k.
         static_for<0, N>([&](auto K) {
@@ -184,8 +189,8 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double 
                     if (i < N) cb[i] = y * myri;
                     asm volatile("" ::: "memory");
                     constexpr int rowk = K * (K - 1) / 2;
-                    const double l = lr[rowk + (iw < K ? iw : 0)];
-                    y = fma(-(iw < K ? l : 0.0), cb[K], y);
+                    const T l = lr[rowk + (iw < K ? iw : 0)];
+                    y = fma(-(iw < K ? l : T(0)), cb[K], y);
                 }
             });
         }
@@ -194,14 +199,15 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, double 
     }
 }
 
-int launch_posv_small(int n, int nrhs, double *A, int64_t lda, int64_t strideA, const double *B, int64_t ldb,
-                      int64_t strideB, double *X, int64_t ldx, int64_t strideX, int *info, int64_t batch,
+template <typename T>
+int launch_posv_small(int n, int nrhs, T *A, int64_t lda, int64_t strideA, const T *B, int64_t ldb,
+                      int64_t strideB, T *X, int64_t ldx, int64_t strideX, int *info, int64_t batch,
                       const int *mask, bool write_l, hipStream_t st) {
     const int64_t per_wave = (n <= 16) ? 4 : 2;
     const int64_t blocks = (batch + per_wave - 1) / per_wave;
     const dim3 grid((unsigned)blocks), block(64);
 #define GPUTILS_SMALL(NV, WL)                                                                                        \
-    hipLaunchKernelGGL((posv_small_kernel<NV, WL, false>), grid, block, 0, st, n, nrhs, A, lda, strideA, B, ldb, strideB, X, \
+    hipLaunchKernelGGL((posv_small_kernel<T, NV, WL, false>), grid, block, 0, st, n, nrhs, A, lda, strideA, B, ldb, strideB, X, \
                        ldx, strideX, info, batch, mask)
     if (n <= 16) {
         if (write_l) GPUTILS_SMALL(16, true);
@@ -214,21 +220,32 @@ int launch_posv_small(int n, int nrhs, double *A, int64_t lda, int64_t strideA, 
     return launch_check("posv_small_kernel");
 }
 
-int launch_potrs_small(int n, int nrhs, const double *L, int64_t lda, int64_t strideL, double *B, int64_t ldb,
+template <typename T>
+int launch_potrs_small(int n, int nrhs, const T *L, int64_t lda, int64_t strideL, T *B, int64_t ldb,
                        int64_t strideB, int *info, int64_t batch, hipStream_t st) {
     const int64_t per_wave = (n <= 16) ? 4 : 2;
     const int64_t blocks = (batch + per_wave - 1) / per_wave;
     const dim3 grid((unsigned)blocks), block(64);
     // L is only read (WRITE_L = false); X aliases B
-    double *l = const_cast<double *>(L);
+    T *l = const_cast<T *>(L);
     if (n <= 16)
-        hipLaunchKernelGGL((posv_small_kernel<16, false, true>), grid, block, 0, st, n, nrhs, l, lda, strideL, B, ldb,
+        hipLaunchKernelGGL((posv_small_kernel<T, 16, false, true>), grid, block, 0, st, n, nrhs, l, lda, strideL, B, ldb,
                            strideB, B, ldb, strideB, info, batch, nullptr);
     else
-        hipLaunchKernelGGL((posv_small_kernel<32, false, true>), grid, block, 0, st, n, nrhs, l, lda, strideL, B, ldb,
+        hipLaunchKernelGGL((posv_small_kernel<T, 32, false, true>), grid, block, 0, st, n, nrhs, l, lda, strideL, B, ldb,
                            strideB, B, ldb, strideB, info, batch, nullptr);
     return launch_check("posv_small_kernel (solve only)");
 }
+
+
+#define GPUTILS_SMALL_INST(T)                                                                                       \
+    template int launch_posv_small<T>(int, int, T *, int64_t, int64_t, const T *, int64_t, int64_t, T *, int64_t,  \
+                                      int64_t, int *, int64_t, const int *, bool, hipStream_t);                   \
+    template int launch_potrs_small<T>(int, int, const T *, int64_t, int64_t, T *, int64_t, int64_t, int *, int64_t, \
+                                       hipStream_t);
+GPUTILS_SMALL_INST(float)
+GPUTILS_SMALL_INST(double)
+#undef GPUTILS_SMALL_INST
 
 }  // namespace chols
 }  // namespace gputils
