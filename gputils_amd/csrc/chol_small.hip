@@ -61,9 +61,11 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, T *A, i
     constexpr int kTri = N * (N - 1) / 2;
     constexpr int G = 64 / N;  // matrices per wavefront, one per group of N lanes
     __shared__ __attribute__((aligned(16))) T LR[G][kTri];  // strict lower L, packed by rows
-    __shared__ __attribute__((aligned(16))) T CB[G][N];     // per-group broadcast buffer
+    // per-group broadcast buffer: columns 0..N-1 carry the factor's column
+    // and the first right-hand side, N..2N-1 L(k, k) during the
+    // factorisation and the second right-hand side during the solves
+    __shared__ __attribute__((aligned(16))) T CB[G][2 * N];
     __shared__ T RI[G][N];                                   // per-group 1 / L(k, k)
-    __shared__ T SD[G][N];                                   // per-group L(k, k)
     const int lane = threadIdx.x, i = lane % N, grp = lane / N;
     const int64_t b = G * (int64_t)blockIdx.x + grp;
     bool active = b < batch;
@@ -73,7 +75,7 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, T *A, i
     T *cb = CB[grp];
     T *lr = LR[grp];
     T *ri = RI[grp];
-    T *sd = SD[grp];
+    T *sd = CB[grp] + N;  // read for the last time before the solves
 
     // every lane loads a valid element (indices clamped into the matrix) so
     // that the loads are unconditional and all in flight together.  Entries
@@ -167,36 +169,52 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, T *A, i
 
     const T *bb = B + (active ? b : 0) * strideB;
     T *xx = X + (active ? b : 0) * strideX;
-    for (int r = 0; r < nrhs; ++r) {
+    // right-hand sides two at a time (y0, y1), each with the same
+    // arithmetic as alone; an odd last one runs the single-column code
+    auto solve = [&](int r, auto PAIR) {
+        constexpr bool two = decltype(PAIR)::value;
         // keeps the LDS reads below inside the loop: hoisted out of it, the
         // loop-invariant reads of L would occupy ~128 VGPRs
         asm volatile("" ::: "memory");
-        T y = row_in ? bb[i + (int64_t)r * ldb] : T(0);
+        T y0 = row_in ? bb[i + (int64_t)r * ldb] : T(0);
+        T y1 = T(0);
+        if constexpr (two) y1 = row_in ? bb[i + (int64_t)(r + 1) * ldb] : T(0);
         // forward: L y = b.  Each lane keeps its unscaled value; lane k's
         // scaled value y_k = y * (1 / L(k, k)) is published at step k.
         static_for<0, N>([&](auto K) {
-            if (i < N) cb[i] = y * myri;
+            cb[i] = y0 * myri;
+            if constexpr (two) cb[N + i] = y1 * myri;
             asm volatile("" ::: "memory");
-            y = fma(-a[K], cb[K], y);
+            y0 = fma(-a[K], cb[K], y0);
+            if constexpr (two) y1 = fma(-a[K], cb[N + K], y1);
         });
-        y *= myri;
+        y0 *= myri;
+        y1 *= myri;
         // backward: L^T x = y; lane i reads L(k, i) from row k of the packed copy
-        {
-            const int iw = opaque(i);
-            static_for<0, N>([&](auto tt) {
-                constexpr int K = N - 1 - tt;
-                if constexpr (K > 0) {
-                    if (i < N) cb[i] = y * myri;
-                    asm volatile("" ::: "memory");
-                    constexpr int rowk = K * (K - 1) / 2;
-                    const T l = lr[rowk + (iw < K ? iw : 0)];
-                    y = fma(-(iw < K ? l : T(0)), cb[K], y);
-                }
-            });
+        const int iw = opaque(i);
+        static_for<0, N>([&](auto tt) {
+            constexpr int K = N - 1 - tt;
+            if constexpr (K > 0) {
+                cb[i] = y0 * myri;
+                if constexpr (two) cb[N + i] = y1 * myri;
+                asm volatile("" ::: "memory");
+                constexpr int rowk = K * (K - 1) / 2;
+                const T l = lr[rowk + (iw < K ? iw : 0)];
+                const T lk = iw < K ? l : T(0);
+                y0 = fma(-lk, cb[K], y0);
+                if constexpr (two) y1 = fma(-lk, cb[N + K], y1);
+            }
+        });
+        y0 *= myri;
+        y1 *= myri;
+        if (ok && i < n) {
+            xx[i + (int64_t)r * ldx] = y0;
+            if constexpr (two) xx[i + (int64_t)(r + 1) * ldx] = y1;
         }
-        y *= myri;
-        if (ok && i < n) xx[i + (int64_t)r * ldx] = y;
-    }
+    };
+    int r = 0;
+    for (; r + 1 < nrhs; r += 2) solve(r, std::true_type{});
+    if (r < nrhs) solve(r, std::false_type{});
 }
 
 template <typename T>
