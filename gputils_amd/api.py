@@ -110,8 +110,11 @@ def posv_mixed_batched(A: torch.Tensor, B: torch.Tensor, X: torch.Tensor | None 
         X = torch.empty_like(B)
     iters = _info(k, A.device, iters)
     info = _info(k, A.device, info)
+    wb = lib().gputils_dposv_mixed_workspace_bytes(n, k)
+    work = torch.empty(wb, dtype=torch.uint8, device=A.device) if wb > 0 else None
     check(lib().gputils_dposv_mixed_batched(n, nrhs, _ptr(A), n, n * n, _ptr(B), n, n * nrhs, _ptr(X), n,
-                                            n * nrhs, tol, max_iter, _ptr(iters), _ptr(info), k,
+                                            n * nrhs, tol, max_iter, _ptr(iters), _ptr(info),
+                                            _ptr(work) if work is not None else None, k,
                                             stream_handle()), "posv_mixed_batched")
     return X, iters, info
 

@@ -5,7 +5,7 @@
 
 Every HIP source under gputils_amd/csrc is compiled with hipcc for gfx950
 (in parallel), then linked into gputils_amd/lib/libgputils_hip.so.  The C++
-drop-in API test (tests/cpp/test_api.cpp, which includes include/tensor.hpp)
+drop-in API suite (tests/cpp/api_suite.cpp, which includes include/tensor.hpp)
 is compiled with g++ and linked against that library.
 """
 from __future__ import annotations
@@ -80,10 +80,10 @@ def build_lib(jobs: int | None = None) -> str:
 
 
 def build_cpp_tests() -> str | None:
-    src = os.path.join(ROOT, "tests", "cpp", "test_api.cpp")
+    src = os.path.join(ROOT, "tests", "cpp", "api_suite.cpp")
     if not os.path.exists(src):
         return None
-    exe = os.path.join(ROOT, "tests", "cpp", "test_api")
+    exe = os.path.join(ROOT, "tests", "cpp", "api_suite")
     cmd = ["g++", "-std=c++17", "-O2", "-Wall", f"-I{INCLUDE}", src, "-o", exe,
            f"-L{LIBDIR}", "-lgputils_hip", f"-Wl,-rpath,{LIBDIR}", "-Wl,-rpath,$ORIGIN/../../gputils_amd/lib"]
     rc, out = _run(cmd)

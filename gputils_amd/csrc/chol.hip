@@ -509,6 +509,19 @@ __global__ __launch_bounds__(kThreads) void potrs_global_kernel(int n, int nrhs,
     if (threadIdx.x == 0 && info != nullptr) info[b] = 0;
 }
 
+// Copies the rows x cols matrices src_b (ld lds) to dst_b (ld ldd).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void copy_batch_kernel(int rows, int cols, const T *__restrict__ src,
+                                                              int64_t lds, int64_t strideS, T *__restrict__ dst,
+                                                              int64_t ldd, int64_t strideD, int64_t batch) {
+    const int64_t per = (int64_t)rows * cols;
+    for (int64_t b = blockIdx.x; b < batch; b += gridDim.x)
+        for (int64_t e = threadIdx.x; e < per; e += kThreads) {
+            const int64_t i = e % rows, j = e / rows;
+            dst[b * strideD + i + j * ldd] = src[b * strideS + i + j * lds];
+        }
+}
+
 // ----------------------------------------------------------- dispatch --
 template <typename T>
 int potrf_dispatch(int64_t n, T *A, int64_t lda, int64_t strideA, int *info, int64_t batch,
@@ -638,18 +651,23 @@ int gputils_dposv_batched(int64_t n, int64_t nrhs, double *A, int64_t lda, int64
                                              strideB, info, batch, nullptr, as_stream(stream));
 }
 
+int64_t gputils_dposv_mixed_workspace_bytes(int64_t n, int64_t batch) {
+    if (n < 0 || batch < 0) return -1;
+    return n <= cholw::kMixedMaxN ? 0 : n * n * (int64_t)sizeof(double) * batch;
+}
+
 int gputils_dposv_mixed_batched(int64_t n, int64_t nrhs, const double *A, int64_t lda,
                                 int64_t strideA, const double *B, int64_t ldb, int64_t strideB,
                                 double *X, int64_t ldx, int64_t strideX, double tol, int max_iter,
-                                int *iters, int *info, int64_t batch, void *stream) {
-    GPUTILS_ARG(n >= 0 && n <= 128, 1, "n must be in [0, 128]");
+                                int *iters, int *info, void *work, int64_t batch, void *stream) {
+    GPUTILS_ARG(n >= 0, 1, "n < 0");
     GPUTILS_ARG(nrhs >= 0, 2, "nrhs < 0");
     GPUTILS_ARG(lda >= (n > 1 ? n : 1), 4, "lda < max(1,n)");
     GPUTILS_ARG(ldb >= (n > 1 ? n : 1), 7, "ldb < max(1,n)");
     GPUTILS_ARG(ldx >= (n > 1 ? n : 1), 10, "ldx < max(1,n)");
     GPUTILS_ARG(tol > 0.0, 12, "tol <= 0");
     GPUTILS_ARG(max_iter >= 0, 13, "max_iter < 0");
-    GPUTILS_ARG(batch >= 0 && batch <= 0x7fffffff, 16, "batch out of range");
+    GPUTILS_ARG(batch >= 0 && batch <= 0x7fffffff, 17, "batch out of range");
     if (n == 0 || nrhs == 0 || batch == 0) return GPUTILS_OK;
     GPUTILS_ARG(A != nullptr, 3, "A is NULL");
     GPUTILS_ARG(B != nullptr, 6, "B is NULL");
@@ -657,6 +675,24 @@ int gputils_dposv_mixed_batched(int64_t n, int64_t nrhs, const double *A, int64_
     GPUTILS_ARG(iters != nullptr, 14, "iters is NULL");
     GPUTILS_ARG(info != nullptr, 15, "info is NULL");
     hipStream_t st = as_stream(stream);
+    if (n > cholw::kMixedMaxN) {
+        // beyond the fused kernel: FP64 factor and solve of copies (A and B
+        // stay read only), every system reported as solved by the fallback
+        GPUTILS_ARG(work != nullptr, 16, "work is NULL (query gputils_dposv_mixed_workspace_bytes)");
+        double *L = static_cast<double *>(work);
+        const unsigned blocks = (unsigned)(batch < 4096 ? batch : 4096);
+        hipLaunchKernelGGL((chol::copy_batch_kernel<double>), dim3(blocks), dim3(chol::kThreads), 0, st, (int)n, (int)n, A,
+                           lda, strideA, L, n, n * n, batch);
+        hipLaunchKernelGGL((chol::copy_batch_kernel<double>), dim3(blocks), dim3(chol::kThreads), 0, st, (int)n, (int)nrhs,
+                           B, ldb, strideB, X, ldx, strideX, batch);
+        int rc = launch_check("copy_batch_kernel");
+        if (rc != GPUTILS_OK) return rc;
+        rc = gputils_fill_bytes_async(iters, 0xff, (size_t)batch * sizeof(int), stream);  // iters = -1
+        if (rc != GPUTILS_OK) return rc;
+        rc = chol::potrf_dispatch<double>(n, L, n, n * n, info, batch, st);
+        if (rc != GPUTILS_OK) return rc;
+        return chol::potrs_dispatch<double>(n, nrhs, L, n, n * n, X, ldx, strideX, nullptr, batch, st);
+    }
     int rc = cholw::launch_posv_mixed((int)n, (int)nrhs, A, lda, strideA, B, ldb, strideB, X, ldx, strideX, tol,
                                       max_iter, iters, info, batch, st);
     if (rc != GPUTILS_OK) return rc;

@@ -24,6 +24,7 @@
 //   ||r||_inf <= tol (||A||_inf ||x||_inf + ||b||_inf).
 // Right-hand sides are processed four at a time (columns 0..3 of a 16x16
 // vector block; the other columns are zero).
+#include <cfloat>
 #include <type_traits>
 
 #include "chol_mixed.hpp"
@@ -497,8 +498,12 @@ __device__ __forceinline__ void residual_padded(const double *__restrict__ a, in
 
 // m[q] = max over rows of |V(row, q)| for a [row][4] LDS array of NP rows
 // (uniform result).
+// Column maxima of |V| over the wavefront.  fmax drops NaN operands, so
+// the return value says separately whether every entry was finite (one
+// ballot over the lanes' flags, uniform across the wavefront).
 template <int NP>
-__device__ __forceinline__ void col_absmax(const double *V, int ln, double (&m)[kRhs]) {
+__device__ __forceinline__ bool col_absmax(const double *V, int ln, double (&m)[kRhs]) {
+    bool bad = false;
 #pragma unroll
     for (int q = 0; q < kRhs; ++q) m[q] = 0.0;
 #pragma unroll
@@ -507,11 +512,16 @@ __device__ __forceinline__ void col_absmax(const double *V, int ln, double (&m)[
         if (row0 + 64 <= NP || row < NP) {
             const double *v = V + 4 * row;
 #pragma unroll
-            for (int q = 0; q < kRhs; ++q) m[q] = fmax(m[q], fabs(v[q]));
+            for (int q = 0; q < kRhs; ++q) {
+                const double a = fabs(v[q]);
+                bad = bad || !(a <= DBL_MAX);  // NaN or infinity
+                m[q] = fmax(m[q], a);
+            }
         }
     }
 #pragma unroll
     for (int q = 0; q < kRhs; ++q) m[q] = dpp_wave_max(m[q]);
+    return __ballot(bad) == 0;
 }
 
 template <int NB>
@@ -606,9 +616,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void po
                 if (anorm == 0.0) anorm = 1e-300;  // A = 0 is not SPD; keep the test finite
             }
             double rm[kRhs], xm[kRhs];
-            col_absmax<NP>(RD, ln, rm);
-            col_absmax<NP>(XD, ln, xm);
-            bool ok = true;
+            // a residual or iterate that is not finite (overflow of the FP32
+            // solve, NaN or infinity in A or b) never passes the test: the
+            // system goes to the FP64 fallback, as LAPACK DSPOSV does
+            const bool finite_r = col_absmax<NP>(RD, ln, rm);
+            const bool finite_x = col_absmax<NP>(XD, ln, xm);
+            const bool finite = finite_r && finite_x;
+            bool ok = finite && anorm <= DBL_MAX;
 #pragma unroll
             for (int q = 0; q < kRhs; ++q)
                 if (q < nr) ok = ok && (rm[q] <= tol * (anorm * xm[q] + bn[q]));

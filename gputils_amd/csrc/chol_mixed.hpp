@@ -9,7 +9,10 @@
 namespace gputils {
 namespace cholw {
 
-// Launches one wavefront per matrix (n <= 128).  iters[b] receives the
+// Largest n of the fused mixed-precision kernel.
+constexpr int kMixedMaxN = 128;
+
+// Launches one wavefront per matrix (n <= kMixedMaxN).  iters[b] receives the
 // number of refinement steps, or -1 when the FP32 factorisation failed or
 // the refinement did not reach tol within max_iter steps.
 int launch_posv_mixed(int n, int nrhs, const double *A, int64_t lda, int64_t strideA, const double *B, int64_t ldb,

@@ -25,6 +25,13 @@
 //   4. U = Q diag(U_R, I_{m-n}): for m <= 128 all columns at once in
 //      registers (eight per 16-lane group, eight rows per lane); otherwise up
 //      to 16 columns at a time with one 16-lane group per column.
+//
+// Where the matrices live (the kernel's MODE): kResident keeps G, W and V
+// in LDS; kGlobalG moves G to the global workspace when it does not fit;
+// kWide (n > 64, or m too large for either) keeps G, W and V in the global
+// workspace and LDS holds only the per-column scalars.  kResident and
+// kGlobalG hold each Jacobi pair's column entries in registers (n <= 64);
+// kWide loops over them.
 #include <algorithm>
 #include <cfloat>
 
@@ -37,45 +44,89 @@ namespace svd {
 using hh::kThreads;
 using hh::wave_sync;
 
-constexpr int kMaxN = 64;
+constexpr int kMaxN = 64;  // largest n whose Jacobi pairs are held in registers
 constexpr int kMaxSweeps = 40;
 constexpr int kMaxLds = 160 * 1024;
 
-// Dynamic LDS layout, in doubles unless noted.
+constexpr int kResident = 0;  // G, W, V in LDS
+constexpr int kGlobalG = 1;   // G in the workspace, W, V in LDS
+constexpr int kWide = 2;      // G, W, V in the workspace
+
+// Storage layout, in doubles unless noted: offsets into LDS, or (kWide, and
+// G of kGlobalG) into the matrix's slice of the workspace.
 struct Layout {
-    int g;      // G (m x n), resident path only
-    int w;      // W (n x n) during Jacobi, then a chunk of U (m x ch)
-    int ws;     // size of the W region
-    int x;      // V (n x n) during Jacobi, then U_R
-    int sig;    // n
-    int tau;    // n + 2 (two lookahead slots)
-    int sgn;    // n
-    int ints;   // pos (n), ord (n), nonzero count (1) as ints
-    int ch;     // U columns per chunk
-    int bytes;  // total
+    int64_t g;       // G (m x n)
+    int64_t w;       // W (n x n) during Jacobi, then a chunk of U (m x ch)
+    int64_t ws;      // size of the W region
+    int64_t x;       // V (n x n) during Jacobi, then U_R
+    int sig;         // n
+    int tau;         // n + 2 (two lookahead slots)
+    int sgn;         // n
+    int ints;        // pos (n), ord (n), nonzero count (1) as ints
+    int ch;          // U columns per chunk
+    int bytes;       // LDS bytes
+    int64_t wstride; // workspace doubles per matrix
 };
 
-__host__ __device__ inline Layout layout(int m, int n, bool resident) {
+__host__ __device__ inline Layout layout(int m, int n, int mode) {
     Layout L;
-    int o = 0;
-    L.g = o;
-    o += resident ? m * n : 0;
-    L.ws = (n * n > 4 * m) ? n * n : 4 * m;
-    L.w = o;
-    o += L.ws;
-    L.x = o;
-    o += n * n;
-    L.sig = o;
+    const int64_t mn = (int64_t)m * n, nn = (int64_t)n * n;
+    L.ws = (nn > 4 * (int64_t)m) ? nn : 4 * (int64_t)m;
+    int64_t o = 0, gofs = 0;  // LDS and workspace cursors
+    if (mode == kResident) {
+        L.g = o;
+        o += mn;
+    } else {
+        L.g = gofs;
+        gofs += mn;
+    }
+    if (mode == kWide) {
+        L.w = gofs;
+        gofs += L.ws;
+        L.x = gofs;
+        gofs += nn;
+    } else {
+        L.w = o;
+        o += L.ws;
+        L.x = o;
+        o += nn;
+    }
+    L.sig = (int)o;
     o += n;
-    L.tau = o;
+    L.tau = (int)o;
     o += n + 2;
-    L.sgn = o;
+    L.sgn = (int)o;
     o += n;
-    L.ints = o;
+    L.ints = (int)o;
     o += (2 * n + 2) / 2 + 1;
-    L.ch = L.ws / m < 16 ? L.ws / m : 16;
-    L.bytes = o * 8;
+    L.ch = L.ws / m < 16 ? (int)(L.ws / m) : 16;
+    L.bytes = (int)(o * 8 < (int64_t)1 << 30 ? o * 8 : (int64_t)1 << 30);
+    L.wstride = gofs;
     return L;
+}
+
+// Rotation that makes columns p and q orthogonal, from al = |w_p|^2,
+// be = |w_q|^2 and ga = w_p . w_q (ga != 0):
+// t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)) with zeta = (be - al) / (2 ga),
+// multiplied through by 2 |ga|: one square root and one division, both
+// from the hardware estimates with one Newton step each (t only sets the
+// angle; cs, sn form a rotation for any t, accurate to a few ulp).
+__device__ __forceinline__ void jacobi_angle(double al, double be, double ga, double &cs, double &sn) {
+    const double d = be - al, g2 = 2.0 * fabs(ga);
+    const double sg = (d == 0.0 || ((d > 0.0) == (ga > 0.0))) ? 1.0 : -1.0;
+    const double h = fma(d, d, g2 * g2);
+    double rh = __builtin_amdgcn_rsq(h);
+    rh = rh * fma(-0.5 * h, rh * rh, 1.5);
+    const double den = fma(h, rh, fabs(d));  // |d| + sqrt(h)
+    double inv = __builtin_amdgcn_rcp(den);
+    inv = inv * fma(-den, inv, 2.0);
+    const double t = sg * g2 * inv;
+    // cs = 1 / sqrt(1 + t^2): hardware estimate and one Newton step
+    // (relative error ~ the estimate's squared)
+    const double t2p1 = fma(t, t, 1.0);
+    cs = __builtin_amdgcn_rsq(t2p1);
+    cs = cs * fma(-0.5 * t2p1, cs * cs, 1.5);
+    sn = cs * t;
 }
 
 // x + (x of the lane that DPP control CTRL selects), within 16-lane rows
@@ -116,7 +167,79 @@ __device__ __forceinline__ void tournament_pair(int i, int r, int n1, int &p, in
     q = a < b ? b : a;
 }
 
-template <typename T, bool RESIDENT>
+// One Jacobi rotation of the column pair (p, q) by the 8 lanes gl = 0..7
+// of a group, for any n: entries i = gl + 8 u are read, summed by
+// group8_sum and rotated in place (W and V columns).  Returns whether the
+// pair needed a rotation (uniform over the group).
+__device__ __forceinline__ bool jacobi_pair_strided(double *wp, double *wq, double *vp, double *vq, int n, int gl,
+                                                    double tol2) {
+    double al = 0.0, be = 0.0, ga = 0.0;
+    for (int i = gl; i < n; i += 8) {
+        const double x = wp[i], y = wq[i];
+        al = fma(x, x, al);
+        be = fma(y, y, be);
+        ga = fma(x, y, ga);
+    }
+    al = group8_sum(al);
+    be = group8_sum(be);
+    ga = group8_sum(ga);
+    if (!(ga * ga > tol2 * al * be)) return false;
+    double cs, sn;
+    jacobi_angle(al, be, ga, cs, sn);
+    for (int i = gl; i < n; i += 8) {
+        const double x = wp[i], y = wq[i], xv = vp[i], yv = vq[i];
+        wp[i] = cs * x - sn * y;
+        wq[i] = sn * x + cs * y;
+        vp[i] = cs * xv - sn * yv;
+        vq[i] = sn * xv + cs * yv;
+    }
+    return true;
+}
+
+// Completion of X's columns nz..n-1 (numerically zero sigma) by one
+// wavefront, for any n: column col is the first unit vector e_k (k from
+// `next` on) whose residual against columns 0..col-1 (two Gram-Schmidt
+// passes, worked in place in column col) exceeds 1/2, normalised; if none
+// does, the one with the largest residual.
+__device__ void complete_basis_strided(double *X, int n, int nz, int lane) {
+    int next = 0;
+    for (int col = nz; col < n; ++col) {
+        double *x = X + (int64_t)col * n;
+        auto residual_of = [&](int k) {
+            for (int i = lane; i < n; i += 64) x[i] = (i == k) ? 1.0 : 0.0;
+            wave_sync();
+            for (int pass = 0; pass < 2; ++pass)
+                for (int c = 0; c < col; ++c) {
+                    const double *uc = X + (int64_t)c * n;
+                    double part = 0.0;
+                    for (int i = lane; i < n; i += 64) part = fma(uc[i], x[i], part);
+                    const double coef = wave_sum(part);
+                    for (int i = lane; i < n; i += 64) x[i] -= coef * uc[i];
+                    wave_sync();
+                }
+            double sq = 0.0;
+            for (int i = lane; i < n; i += 64) sq = fma(x[i], x[i], sq);
+            return sqrt(wave_sum(sq));
+        };
+        if (next >= n) next = 0;  // every unit vector used: search them all again
+        double best_norm = -1.0, nrm = -1.0;
+        int best_k = next;
+        for (int k = next; k < n; ++k) {
+            nrm = residual_of(k);
+            if (nrm > best_norm) {
+                best_norm = nrm;
+                best_k = k;
+            }
+            if (nrm > 0.5) break;
+        }
+        if (!(nrm > 0.5) && best_k != n - 1) best_norm = residual_of(best_k);  // the best was overwritten
+        for (int i = lane; i < n; i += 64) x[i] /= best_norm;
+        wave_sync();
+        if (best_norm > 0.5) next = best_k + 1;
+    }
+}
+
+template <typename T, int MODE>
 __global__ __launch_bounds__(kThreads) void gesvd_kernel(int m, int n, const T *__restrict__ A, int64_t lda,
                                                          int64_t strideA, T *S, int64_t strideS, T *U,
                                                          int64_t ldu, int64_t strideU, T *VT, int64_t ldvt,
@@ -125,11 +248,12 @@ __global__ __launch_bounds__(kThreads) void gesvd_kernel(int m, int n, const T *
     extern __shared__ double lds[];
     const int64_t b = blockIdx.x;
     if (b >= batch) return;
-    const Layout L = layout(m, n, RESIDENT);
+    const Layout L = layout(m, n, MODE);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    double *G = RESIDENT ? lds + L.g : work + b * (int64_t)m * n;
-    double *W = lds + L.w;
-    double *X = lds + L.x;
+    double *ws = work + b * L.wstride;  // this matrix's workspace (unused by kResident)
+    double *G = MODE == kResident ? lds + L.g : ws + L.g;
+    double *W = MODE == kWide ? ws + L.w : lds + L.w;
+    double *X = MODE == kWide ? ws + L.x : lds + L.x;
     double *sig = lds + L.sig;
     double *tau = lds + L.tau;
     double *s_tau = tau + n;
@@ -170,7 +294,15 @@ __global__ __launch_bounds__(kThreads) void gesvd_kernel(int m, int n, const T *
         ++sweeps;
         int rot = 0;
         for (int r = 0; r < n1; ++r) {
-            if (grp < npairs) {
+            if constexpr (MODE == kWide) {
+                for (int pr = grp; pr < npairs; pr += kThreads / 8) {
+                    int p, q;
+                    tournament_pair(pr, r, n1, p, q);
+                    if (q < n && jacobi_pair_strided(W + (int64_t)p * n, W + (int64_t)q * n, X + (int64_t)p * n,
+                                                     X + (int64_t)q * n, n, gl, tol2))
+                        rot = 1;
+                }
+            } else if (grp < npairs) {
                 int p, q;
                 tournament_pair(grp, r, n1, p, q);
                 if (q < n) {
@@ -203,28 +335,8 @@ __global__ __launch_bounds__(kThreads) void gesvd_kernel(int m, int n, const T *
                     // |ga| > tol sqrt(al be), squared: no square roots on the
                     // round's critical path
                     if (ga * ga > tol2 * al * be) {
-                        // t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)) with
-                        // zeta = (be - al) / (2 ga), multiplied through by
-                        // 2 |ga|: one square root and one division
-                        const double d = be - al, g2 = 2.0 * fabs(ga);
-                        const double sg = (d == 0.0 || ((d > 0.0) == (ga > 0.0))) ? 1.0 : -1.0;
-                        // the root and the quotient from the hardware
-                        // estimates with one Newton step each: t only sets
-                        // the angle (cs, sn below form a rotation for any
-                        // t), and these are accurate to a few ulp
-                        const double h = fma(d, d, g2 * g2);
-                        double rh = __builtin_amdgcn_rsq(h);
-                        rh = rh * fma(-0.5 * h, rh * rh, 1.5);
-                        const double den = fma(h, rh, fabs(d));  // |d| + sqrt(h)
-                        double inv = __builtin_amdgcn_rcp(den);
-                        inv = inv * fma(-den, inv, 2.0);
-                        const double t = sg * g2 * inv;
-                        // cs = 1 / sqrt(1 + t^2): hardware estimate and one
-                        // Newton step (relative error ~ the estimate's squared)
-                        const double t2p1 = fma(t, t, 1.0);
-                        double cs = __builtin_amdgcn_rsq(t2p1);
-                        cs = cs * fma(-0.5 * t2p1, cs * cs, 1.5);
-                        const double sn = cs * t;
+                        double cs, sn;
+                        jacobi_angle(al, be, ga, cs, sn);
                         double *vp = X + p * n, *vq = X + q * n;
                         double xv[kPer], yv[kPer];
 #pragma unroll
@@ -261,8 +373,7 @@ __global__ __launch_bounds__(kThreads) void gesvd_kernel(int m, int n, const T *
         if (gl == 0) sig[j] = sqrt(s);
     }
     __syncthreads();
-    if (tid < n) {
-        const int j = tid;
+    for (int j = tid; j < n; j += kThreads) {
         const double sj = sig[j];
         double smax = 0.0;
         int ps = 0;
@@ -326,7 +437,9 @@ __global__ __launch_bounds__(kThreads) void gesvd_kernel(int m, int n, const T *
         }
     }
     __syncthreads();
-    if (wave == 0 && nz < n) {
+    if (MODE == kWide) {
+        if (wave == 0 && nz < n) complete_basis_strided(X, n, nz, lane);
+    } else if (wave == 0 && nz < n) {
         int next = 0;
         for (int col = nz; col < n; ++col) {
             double best_x = 0.0, best_norm = -1.0;
@@ -530,14 +643,20 @@ __global__ __launch_bounds__(kThreads) void nullspace_build_kernel(int n, const 
     }
 }
 
-inline bool resident_fits(int m, int n) { return layout(m, n, true).bytes <= kMaxLds; }
+// Where a matrix of m x n lives: in LDS when it fits, G in the workspace
+// next, everything in the workspace for n > kMaxN or very tall matrices.
+inline int gesvd_mode(int m, int n) {
+    if (n <= kMaxN && layout(m, n, kResident).bytes <= kMaxLds) return kResident;
+    if (n <= kMaxN && layout(m, n, kGlobalG).bytes <= kMaxLds) return kGlobalG;
+    return kWide;
+}
 
-template <typename T, bool RES>
+template <typename T, int MODE>
 int launch_gesvd(int m, int n, const T *A, int64_t lda, int64_t strideA, T *S, int64_t strideS, T *U,
                  int64_t ldu, int64_t strideU, T *VT, int64_t ldvt, int64_t strideVT, double *work, int *info,
                  int *sweeps, int64_t batch, hipStream_t st) {
-    const Layout L = layout(m, n, RES);
-    auto kern = gesvd_kernel<T, RES>;
+    const Layout L = layout(m, n, MODE);
+    auto kern = gesvd_kernel<T, MODE>;
     if (L.bytes > 64 * 1024)
         GPUTILS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                         hipFuncAttributeMaxDynamicSharedMemorySize, L.bytes));
@@ -550,7 +669,7 @@ template <typename T>
 int gesvd_batched(int m, int n, const T *A, int64_t lda, int64_t strideA, T *S, int64_t strideS, T *U,
                   int64_t ldu, int64_t strideU, T *VT, int64_t ldvt, int64_t strideVT, void *work, int *info,
                   int *sweeps, int64_t batch, void *stream) {
-    GPUTILS_ARG(n >= 1 && n <= kMaxN, 2, "n must be in [1, 64]");
+    GPUTILS_ARG(n >= 1, 2, "n must be >= 1");
     GPUTILS_ARG(m >= n, 1, "m must be >= n (tall matrices; transpose a fat one first)");
     GPUTILS_ARG(lda >= m, 4, "lda < m");
     GPUTILS_ARG(strideA >= 0, 5, "negative strideA");
@@ -565,12 +684,16 @@ int gesvd_batched(int m, int n, const T *A, int64_t lda, int64_t strideA, T *S, 
     GPUTILS_ARG(A != nullptr, 3, "A is NULL");
     GPUTILS_ARG(S != nullptr, 6, "S is NULL");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    if (resident_fits(m, n))
-        return launch_gesvd<T, true>(m, n, A, lda, strideA, S, strideS, U, ldu, strideU, VT, ldvt, strideVT,
-                                     nullptr, info, sweeps, batch, st);
-    GPUTILS_ARG(layout(m, n, false).bytes <= kMaxLds, 1, "m too large for the batched SVD");
+    const int mode = gesvd_mode(m, n);
+    if (mode == kResident)
+        return launch_gesvd<T, kResident>(m, n, A, lda, strideA, S, strideS, U, ldu, strideU, VT, ldvt, strideVT,
+                                          nullptr, info, sweeps, batch, st);
+    GPUTILS_ARG(layout(m, n, mode).bytes <= kMaxLds, 1, "m too large for the batched SVD");
     GPUTILS_ARG(work != nullptr, 14, "work is NULL (query gputils_gesvd_workspace_bytes)");
-    return launch_gesvd<T, false>(m, n, A, lda, strideA, S, strideS, U, ldu, strideU, VT, ldvt, strideVT,
+    if (mode == kGlobalG)
+        return launch_gesvd<T, kGlobalG>(m, n, A, lda, strideA, S, strideS, U, ldu, strideU, VT, ldvt, strideVT,
+                                         static_cast<double *>(work), info, sweeps, batch, st);
+    return launch_gesvd<T, kWide>(m, n, A, lda, strideA, S, strideS, U, ldu, strideU, VT, ldvt, strideVT,
                                   static_cast<double *>(work), info, sweeps, batch, st);
 }
 
@@ -623,9 +746,10 @@ using namespace gputils;
 extern "C" {
 
 int64_t gputils_gesvd_workspace_bytes(int m, int n, int64_t batch) {
-    if (m < n || n < 1 || n > svd::kMaxN || batch < 0) return -1;
-    if (svd::resident_fits(m, n)) return 0;
-    return (int64_t)m * n * (int64_t)sizeof(double) * batch;
+    if (m < n || n < 1 || batch < 0) return -1;
+    const int mode = svd::gesvd_mode(m, n);
+    if (svd::layout(m, n, mode).bytes > svd::kMaxLds) return -1;
+    return svd::layout(m, n, mode).wstride * (int64_t)sizeof(double) * batch;
 }
 
 int gputils_sgesvd_batched(int m, int n, const float *A, int64_t lda, int64_t strideA, float *S, int64_t strideS,

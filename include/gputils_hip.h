@@ -167,15 +167,19 @@ int gputils_dposv_batched(int64_t n, int64_t nrhs, double *A, int64_t lda, int64
  *   ||b - A x||_inf <= tol * (||A||_inf ||x||_inf + ||b||_inf),
  * or max_iter refinement steps were made.  A_i and B_i are read only
  * (FP64, lower triangle of A_i used); X_i receives the FP64 solution.
- * iters[i] = refinement steps used, or -1 if the FP32 factorisation failed
- * or refinement did not converge; those matrices are then solved by a
- * second, FP64 launch (as LAPACK DSPOSV falls back), which also reads A_i
- * and B_i only, and info[i] holds the FP64 status (0 or the failing minor).
- * Requires n <= 128. */
+ * iters[i] = refinement steps used, or -1 if the FP32 factorisation failed,
+ * refinement did not converge, or a norm in the stopping test was not
+ * finite; those matrices are then solved by a second, FP64 launch (as
+ * LAPACK DSPOSV falls back), which also reads A_i and B_i only, and
+ * info[i] holds the FP64 status (0 or the failing minor).
+ * n <= 128 runs the fused mixed kernel (work may be NULL).  Larger n is
+ * solved in FP64 only (iters[i] = -1 for all i) on a copy of A in `work`,
+ * gputils_dposv_mixed_workspace_bytes(n, batch) bytes of device memory. */
+int64_t gputils_dposv_mixed_workspace_bytes(int64_t n, int64_t batch);
 int gputils_dposv_mixed_batched(int64_t n, int64_t nrhs, const double *A, int64_t lda,
                                 int64_t strideA, const double *B, int64_t ldb, int64_t strideB,
                                 double *X, int64_t ldx, int64_t strideX, double tol, int max_iter,
-                                int *iters, int *info, int64_t batch, void *stream);
+                                int *iters, int *info, void *work, int64_t batch, void *stream);
 
 /* ------------------------------------------------------- QR / least sq --- */
 /* Batched least squares min ||A_i x - b_i|| for m >= n (cublas?gelsBatched

@@ -6,6 +6,13 @@
  * C++17 compiler (no HIP/CUDA compiler needed) and links
  * libgputils_hip.so, whose C ABI (gputils_hip.h) runs the gfx950 kernels.
  *
+ * Storage model.  A DTensor is a view (shape, first element, per-matrix
+ * pointer table, stream index) plus up to two owned device buffers
+ * (DeviceBuffer: one counted allocation, freed with its owner).  A tensor
+ * that allocated its elements owns them; slices and operator= targets only
+ * view someone else's.  The same holds for the pointer table: an axis-2
+ * slice borrows the parent's table (shifted by its first matrix).
+ *
  * Deliberate differences from tensor.cuh (each one documented at the
  * method):
  *  - Session exposes HIP-native names (stream(), allocate()) instead of the
@@ -14,15 +21,23 @@
  *  - CholeskyBatchFactoriser::solve accepts b with any number of columns.
  *  - Svd::rank() recomputes the rank on each call instead of adding to a
  *    counter that is never reset.
- *  - Svd handles at most 64 columns and never modifies the factorised
- *    matrix (cuSOLVER's gesvd overwrites it).
- *  - reshape() keeps the allocated-bytes counter exact.
- *  - StreamGraph (new) captures repeated calls into a HIP graph.
+ *  - Svd never modifies the factorised matrix (cuSOLVER's gesvd
+ *    overwrites it); Nullspace throws if an SVD did not converge.
+ *  - Slices check their range; reshape() keeps the allocated-bytes counter
+ *    exact and gives a reshaped slice a pointer table of its new shape.
+ *  - operator= releases the storage the target owned before it aliases
+ *    (the reference keeps counting it and never frees it), and assigning
+ *    from a temporary moves its storage instead of aliasing memory that is
+ *    freed at the end of the statement.
+ *  - New, with no reference counterpart: MixedCholeskyBatchSolver (FP32
+ *    factor + FP64 refinement), RaggedLeastSquares (uneven sizes in one
+ *    plan), addAB(..., GemmMode::bf16x3) and StreamGraph (HIP graphs).
  */
 #ifndef GPUTILS_TENSOR_HPP
 #define GPUTILS_TENSOR_HPP
 
 #include <algorithm>
+#include <cerrno>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -31,13 +46,16 @@
 #include <fstream>
 #include <iomanip>
 #include <iostream>
-#include <iterator>
+#include <limits>
 #include <memory>
+#include <numeric>
 #include <optional>
 #include <random>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "gputils_hip.h"
@@ -49,30 +67,36 @@
 
 /* ------------------------------------------------------------ helpers --- */
 
-static std::random_device RND_DEVICE;
+namespace gputils_detail {
+/** One process-wide engine for the unseeded helpers below. */
+inline std::mt19937_64 &hostEngine() {
+    static std::mt19937_64 engine{std::random_device{}()};
+    return engine;
+}
+}  // namespace gputils_detail
 
-/** Vector of n uniform reals in [low, hi] (tensor.cuh:47-58). */
+/** n uniform reals in [low, hi], unseeded (tensor.cuh:47-58). */
 TEMPLATE_WITH_TYPE_T
 std::vector<T> generateRealRandomVector(size_t n, T low, T hi) {
-    std::mt19937_64 engine(RND_DEVICE());
-    std::uniform_real_distribution<T> dist(low, hi);
-    std::vector<T> vec(n);
-    for (auto &v : vec) v = dist(engine);
-    return vec;
+    std::uniform_real_distribution<T> draw(low, hi);
+    std::vector<T> out;
+    out.reserve(n);
+    while (out.size() < n) out.push_back(draw(gputils_detail::hostEngine()));
+    return out;
 }
 
-/** Vector of n uniform integers in [low, hi] (tensor.cuh:60-69). */
+/** n uniform integers in [low, hi], unseeded (tensor.cuh:60-69). */
 inline std::vector<int> generateIntRandomVector(size_t n, int low, int hi) {
-    std::mt19937_64 engine(RND_DEVICE());
-    std::uniform_int_distribution<int> dist(low, hi);
-    std::vector<int> vec(n);
-    for (auto &v : vec) v = dist(engine);
-    return vec;
+    std::uniform_int_distribution<int> draw(low, hi);
+    std::vector<int> out;
+    out.reserve(n);
+    while (out.size() < n) out.push_back(draw(gputils_detail::hostEngine()));
+    return out;
 }
 
-/** Ceiling division (tensor.cuh:79-81). */
+/** Blocks of threads_per_block threads needed for n items (tensor.cuh:79-81). */
 constexpr size_t numBlocks(size_t n, size_t threads_per_block = THREADS_PER_BLOCK) {
-    return (n / threads_per_block + (n % threads_per_block != 0));
+    return (n + threads_per_block - 1) / threads_per_block;
 }
 
 /** Abort with a message on a failing C-ABI status (tensor.cuh:87-113). */
@@ -114,6 +138,9 @@ public:
         if (idx >= m_streams.size()) throw std::invalid_argument("stream index out of range");
         return m_streams[idx];
     }
+
+    /** Number of streams this session holds. */
+    size_t numStreams() const { return m_streams.size(); }
 
     /** Counted device allocation (tensor.cuh:209-215, HIP-native name). */
     int allocate(void **d, size_t s) {
@@ -168,7 +195,7 @@ private:
     size_t m_bytesAllocated = 0;
 };
 
-/* ------------------------------------------------------------- tensor --- */
+/* ---------------------------------------------------- device storage --- */
 
 enum StorageMode {
     columnMajor,
@@ -184,93 +211,174 @@ template <typename T>
 inline void require_fp() {
     static_assert(is_fp<T>, "this DTensor operation needs T = float or double");
 }
+
+/**
+ * One device allocation counted in Session's byte total and released by
+ * its destructor.  Movable, not copyable.  Copies into and out of device
+ * memory go through the null stream after every Session stream has
+ * drained, the ordering of the reference's synchronous cudaMemcpy.
+ */
+class DeviceBuffer {
+public:
+    DeviceBuffer() = default;
+    DeviceBuffer(size_t bytes, bool zero) {
+        if (bytes == 0) return;
+        gpuErrChk(Session::getInstance().allocate(&m_ptr, bytes));
+        m_bytes = bytes;
+        if (zero) gpuErrChk(gputils_memset(m_ptr, 0, bytes));
+    }
+    ~DeviceBuffer() { release(); }
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    DeviceBuffer(DeviceBuffer &&from) noexcept
+        : m_ptr(std::exchange(from.m_ptr, nullptr)), m_bytes(std::exchange(from.m_bytes, 0)) {}
+    DeviceBuffer &operator=(DeviceBuffer &&from) noexcept {
+        if (this != &from) {
+            release();
+            m_ptr = std::exchange(from.m_ptr, nullptr);
+            m_bytes = std::exchange(from.m_bytes, 0);
+        }
+        return *this;
+    }
+
+    void *get() const { return m_ptr; }
+    size_t bytes() const { return m_bytes; }
+    bool empty() const { return m_ptr == nullptr; }
+    /** Whether address p lies inside this allocation. */
+    bool holds(const void *p) const {
+        const char *base = static_cast<const char *>(m_ptr), *q = static_cast<const char *>(p);
+        return m_ptr != nullptr && q >= base && q < base + m_bytes;
+    }
+
+    void release() {
+        if (!m_ptr) return;
+        gpuErrChk(gputils_free(m_ptr));
+        Session::getInstance().incrementAllocatedBytes(-static_cast<long long>(m_bytes));
+        m_ptr = nullptr;
+        m_bytes = 0;
+    }
+
+private:
+    void *m_ptr = nullptr;
+    size_t m_bytes = 0;
+};
+
+/** Blocking copy between host and device memory (kind as gputils_memcpy). */
+inline void blockingCopy(void *dst, const void *src, size_t bytes, int kind) {
+    if (bytes == 0) return;
+    Session::getInstance().synchronizeAllStreams();
+    gpuErrChk(gputils_memcpy(dst, src, bytes, kind));
+}
+
+/** Rows x cols x mats extent of a tensor. */
+struct Shape {
+    size_t rows = 0, cols = 0, mats = 0;
+    size_t perMatrix() const { return rows * cols; }
+    size_t count() const { return rows * cols * mats; }
+    bool operator==(const Shape &o) const { return rows == o.rows && cols == o.cols && mats == o.mats; }
+    std::string str() const {
+        return "(" + std::to_string(rows) + ", " + std::to_string(cols) + ", " + std::to_string(mats) + ")";
+    }
+};
+
+/** Row-major (per matrix) host data rearranged into column-major order. */
+template <typename T>
+std::vector<T> rowToColumnMajor(const std::vector<T> &rowMajorData, const Shape &s) {
+    std::vector<T> out(rowMajorData.size());
+    const size_t per = s.perMatrix();
+    for (size_t e = 0; e < out.size(); e++) {
+        const size_t mat = e / per, within = e % per;
+        const size_t row = within % s.rows, col = within / s.rows;  // e is column-major
+        out[e] = rowMajorData[mat * per + row * s.cols + col];
+    }
+    return out;
+}
 }  // namespace gputils_detail
+
+/* ------------------------------------------------------------- tensor --- */
+
+/** Precision of a single-precision batched product (no reference counterpart). */
+enum class GemmMode {
+    standard,  ///< FP32 (or FP64) products and accumulation
+    bf16x3     ///< float only: each operand split into two bfloat16 terms, three MFMA products, FP32 accumulation
+};
 
 /**
  * (m, n, k) tensor of k column-major m x n matrices on the GPU
- * (tensor.cuh:271-661).  Owns its data unless it is a slice or the target of
- * operator= (which aliases).  For k > 1 a device array of per-matrix
- * pointers is kept, as in the reference, so that ptrMatrices() and the byte
- * count match.
+ * (tensor.cuh:271-661).  Element (i, j, l) sits at i + m*(j + n*l).
+ * Owns its elements unless it is a slice or the target of operator=
+ * (which alias).  For k > 1 a device array of per-matrix pointers is kept,
+ * as in the reference, so that ptrMatrices() and the byte count match.
  */
 TEMPLATE_WITH_TYPE_T
 class DTensor {
 private:
-    T *m_d_data = nullptr;
-    T **m_d_ptrMatrices = nullptr;
-    size_t m_numRows = 0;
-    size_t m_numCols = 0;
-    size_t m_numMats = 0;
-    bool m_doDestroyData = false;
-    bool m_doDestroyPtrMatrices = false;
-    size_t m_ptrCapacity = 0;  ///< entries of the owned pointer array
+    using Shape = gputils_detail::Shape;
+    using DeviceBuffer = gputils_detail::DeviceBuffer;
+
+    Shape m_shape;
+    T *m_data = nullptr;           ///< first element (owned or viewed)
+    T **m_table = nullptr;         ///< device table of per-matrix pointers (owned or borrowed)
+    DeviceBuffer m_elements;       ///< non-empty iff this tensor owns its elements
+    DeviceBuffer m_tableStorage;   ///< non-empty iff this tensor owns its pointer table
     size_t m_idxStream = 0;
 
-    void destroy() {
-        if (m_doDestroyData) {
-            if (m_d_data) gpuErrChk(gputils_free(m_d_data));
-            m_d_data = nullptr;
-            Session::getInstance().incrementAllocatedBytes(-(long long)(m_numRows * m_numCols * m_numMats * sizeof(T)));
-            m_doDestroyData = false;
-        }
-        if (m_doDestroyPtrMatrices) {
-            if (m_d_ptrMatrices) gpuErrChk(gputils_free(m_d_ptrMatrices));
-            m_d_ptrMatrices = nullptr;
-            Session::getInstance().incrementAllocatedBytes(-(long long)(m_ptrCapacity * sizeof(T *)));
-            m_doDestroyPtrMatrices = false;
-            m_ptrCapacity = 0;
-        }
+    /** Allocates the elements (and, for several matrices, the pointer table). */
+    void claimStorage(bool zero) {
+        if (m_shape.count() == 0) return;
+        m_elements = DeviceBuffer(m_shape.count() * sizeof(T), zero);
+        m_data = static_cast<T *>(m_elements.get());
+        rebuildPointerTable();
     }
 
-    void allocateOnDevice(size_t size, bool zero = false) {
-        if (size == 0) return;
-        destroy();
-        m_doDestroyData = true;
-        gpuErrChk(Session::getInstance().allocate((void **)&m_d_data, size * sizeof(T)));
-        if (zero) gpuErrChk(gputils_memset(m_d_data, 0, size * sizeof(T)));
-        if (m_numMats > 1) {
-            m_doDestroyPtrMatrices = true;
-            int st = Session::getInstance().allocate((void **)&m_d_ptrMatrices, m_numMats * sizeof(T *));
-            if (st != GPUTILS_OK) {
-                gputils_free(m_d_data);
-                gpuErrChk(st);
-            }
-            m_ptrCapacity = m_numMats;
-        } else {
-            m_doDestroyPtrMatrices = false;
+    /**
+     * Makes the pointer table describe the current shape: for more than one
+     * matrix the table is owned (reused when large enough, else replaced)
+     * and refilled by a kernel; a borrowed table is kept only while the
+     * matrix size and count it was built for still hold.
+     */
+    void rebuildPointerTable(bool borrowedStillValid = false) {
+        if (m_shape.mats <= 1 || borrowedStillValid) return;
+        const size_t need = m_shape.mats * sizeof(T *);
+        if (m_tableStorage.bytes() < need) {
+            m_tableStorage = DeviceBuffer(need, false);
+            m_table = static_cast<T **>(m_tableStorage.get());
+        } else if (m_table != m_tableStorage.get()) {
+            m_table = static_cast<T **>(m_tableStorage.get());
         }
-    }
-
-    void initialisePointersToMatricesData() {
-        if (m_numMats <= 1 || !m_d_ptrMatrices || !m_doDestroyPtrMatrices) return;
-        gpuErrChk(gputils_fill_pointer_array((void **)m_d_ptrMatrices, m_d_data,
-                                             (int64_t)(m_numRows * m_numCols * sizeof(T)),
-                                             (int64_t)m_numMats, stream()));
+        gpuErrChk(gputils_fill_pointer_array(reinterpret_cast<void **>(m_table), m_data,
+                                             static_cast<int64_t>(m_shape.perMatrix() * sizeof(T)),
+                                             static_cast<int64_t>(m_shape.mats), stream()));
         gpuErrChk(gputils_stream_synchronize(stream()));
     }
 
-    void rm2cm(const std::vector<T> &rm, std::vector<T> &cm) const {
-        const size_t n = m_numRows * m_numCols;
-        for (size_t k = 0; k < m_numMats; k++) {
-            const size_t idx = k * n;
-            for (size_t r = 0; r < m_numRows; r++)
-                for (size_t c = 0; c < m_numCols; c++) cm[idx + (r + c * m_numRows)] = rm[idx + (c + r * m_numCols)];
-        }
-    }
-
     std::ostream &print(std::ostream &out) const {
-        out << "Tensor [" << m_numRows << " x " << m_numCols << " x " << m_numMats << "]:" << std::endl;
-        std::vector<T> temp;
-        download(temp);
-        for (size_t k = 0; k < m_numMats; k++) {
-            out << ">> layer: " << k << std::endl;
-            for (size_t i = 0; i < m_numRows; i++) {
-                for (size_t j = 0; j < m_numCols; j++)
-                    out << std::setw(10) << temp[m_numRows * (m_numCols * k + j) + i] << ", ";
-                out << std::endl;
+        std::vector<T> host;
+        download(host);
+        out << "DTensor" << m_shape.str() << ", column-major\n";
+        for (size_t l = 0; l < m_shape.mats; l++) {
+            out << "  matrix " << l << ":\n";
+            for (size_t i = 0; i < m_shape.rows; i++) {
+                out << "   ";
+                for (size_t j = 0; j < m_shape.cols; j++)
+                    out << ' ' << std::setw(12) << host[i + m_shape.rows * (j + m_shape.cols * l)];
+                out << '\n';
             }
         }
         return out;
+    }
+
+    /** this (a fresh copy of `base`) += sign * other; for operator+ and -. */
+    static DTensor combined(const DTensor &base, const DTensor &other, T sign) {
+        gputils_detail::require_fp<T>();
+        DTensor out(base);
+        out.axpyFrom(sign, other);
+        return out;
+    }
+
+    void axpyFrom(T alpha, const DTensor &x) {
+        if constexpr (std::is_same_v<T, double>) gpuErrChk(gputils_daxpy(numEl(), alpha, x.m_data, m_data, stream()));
+        if constexpr (std::is_same_v<T, float>) gpuErrChk(gputils_saxpy(numEl(), alpha, x.m_data, m_data, stream()));
     }
 
     template <typename S>
@@ -280,7 +388,10 @@ public:
     /** Stream handle used by operations on this tensor. */
     void *stream() const { return Session::getInstance().stream(m_idxStream); }
 
-    /** Set the stream index; returns a copy by value like the reference (tensor.cuh:663-670). */
+    /**
+     * Selects the stream that operations on this tensor use; returns a copy
+     * of the tensor by value, as the reference does (tensor.cuh:663-670).
+     */
     DTensor<T> setStreamIdx(size_t idx) {
         if (idx >= s_numStreams) throw std::invalid_argument("Invalid stream index; it exceeds the max allocated streams");
         m_idxStream = idx;
@@ -293,153 +404,122 @@ public:
     static DTensor<T> parseFromFile(std::string path_to_file, StorageMode mode = StorageMode::defaultMajor);
 
     DTensor() = default;
-    ~DTensor() { destroy(); }
+    ~DTensor() = default;
 
     /** (m,n,k) tensor, optionally zeroed (tensor.cuh:863-872). */
-    DTensor(size_t m, size_t n = 1, size_t k = 1, bool zero = false) {
-        m_numRows = m;
-        m_numCols = n;
-        m_numMats = k;
-        allocateOnDevice(m * n * k, zero);
-        initialisePointersToMatricesData();
-    }
+    DTensor(size_t m, size_t n = 1, size_t k = 1, bool zero = false) : m_shape{m, n, k} { claimStorage(zero); }
 
-    /** (m,n,k) tensor uploaded from host data (tensor.cuh:874-884). */
+    /** (m,n,k) tensor holding `data`, given in column- or row-major order (tensor.cuh:874-884). */
     DTensor(const std::vector<T> &data, size_t m, size_t n = 1, size_t k = 1,
-            StorageMode mode = StorageMode::defaultMajor) {
-        m_numRows = m;
-        m_numCols = n;
-        m_numMats = k;
-        allocateOnDevice(m * n * k);
+            StorageMode mode = StorageMode::defaultMajor)
+        : m_shape{m, n, k} {
+        claimStorage(false);
         upload(data, mode);
-        initialisePointersToMatricesData();
     }
 
-    /** Deep copy (tensor.cuh:886-897). */
-    DTensor(const DTensor &other) {
-        m_numMats = other.m_numMats;
-        m_numRows = other.m_numRows;
-        m_numCols = other.m_numCols;
-        m_idxStream = other.m_idxStream;
-        allocateOnDevice(m_numRows * m_numCols * m_numMats);
-        Session::getInstance().synchronizeAllStreams();
-        gpuErrChk(gputils_memcpy(m_d_data, other.raw(), numEl() * sizeof(T), 2));
-        initialisePointersToMatricesData();
+    /** Deep copy on the device (tensor.cuh:886-897). */
+    DTensor(const DTensor &other) : m_shape(other.m_shape), m_idxStream(other.m_idxStream) {
+        claimStorage(false);
+        gputils_detail::blockingCopy(m_data, other.m_data, numEl() * sizeof(T), 2);
     }
 
-    /** Move (tensor.cuh:927-946). */
-    DTensor(DTensor &&other) {
-        m_numCols = other.m_numCols;
-        m_numRows = other.m_numRows;
-        m_numMats = other.m_numMats;
-        m_d_data = other.m_d_data;
-        m_doDestroyData = other.m_doDestroyData;
-        m_doDestroyPtrMatrices = other.m_doDestroyPtrMatrices;
-        m_d_ptrMatrices = other.m_d_ptrMatrices;
-        m_ptrCapacity = other.m_ptrCapacity;
-        m_idxStream = other.m_idxStream;
-        other.m_ptrCapacity = 0;
-        other.m_doDestroyPtrMatrices = false;
-        other.m_doDestroyData = false;
-        other.m_d_ptrMatrices = nullptr;
-        other.m_d_data = nullptr;
-        other.m_numCols = 0;
-        other.m_numRows = 0;
-        other.m_numMats = 0;
-    }
+    /** Takes over other's storage and views; other is left empty (tensor.cuh:927-946). */
+    DTensor(DTensor &&other) noexcept
+        : m_shape(std::exchange(other.m_shape, Shape{})),
+          m_data(std::exchange(other.m_data, nullptr)),
+          m_table(std::exchange(other.m_table, nullptr)),
+          m_elements(std::move(other.m_elements)),
+          m_tableStorage(std::move(other.m_tableStorage)),
+          m_idxStream(other.m_idxStream) {}
 
     /**
-     * Slice (tensor.cuh:899-925), `to` inclusive.  axis 2: matrices
-     * from..to (shares the parent's pointer array); axis 1: columns of the
-     * first matrix; axis 0: flat element range as a column vector.
+     * View of part of `parent` (tensor.cuh:899-925), `to` inclusive.
+     * axis 2: matrices from..to, borrowing the parent's pointer table;
+     * axis 1: columns from..to of the first matrix; axis 0: elements
+     * from..to of the flat element array, as a column vector.  Ranges past
+     * the parent's extent throw (the reference does not check).
      */
-    DTensor(const DTensor &other, size_t axis, size_t from, size_t to) {
+    DTensor(const DTensor &parent, size_t axis, size_t from, size_t to) : m_idxStream(parent.m_idxStream) {
         if (from > to) throw std::invalid_argument("from > to");
-        size_t offset = 0, len = to - from + 1;
-        m_d_ptrMatrices = nullptr;
-        if (axis == 2) {
-            offset = other.m_numRows * other.m_numCols * from;
-            m_numRows = other.m_numRows;
-            m_numCols = other.m_numCols;
-            m_numMats = len;
-            m_d_ptrMatrices = other.m_d_ptrMatrices ? other.m_d_ptrMatrices + from : nullptr;
-        } else if (axis == 1) {
-            offset = other.m_numRows * from;
-            m_numRows = other.m_numRows;
-            m_numCols = len;
-            m_numMats = 1;
-        } else if (axis == 0) {
-            offset = from;
-            m_numRows = len;
-            m_numCols = 1;
-            m_numMats = 1;
+        const size_t span = to - from + 1;
+        const Shape &p = parent.m_shape;
+        switch (axis) {
+            case 2:
+                if (to >= p.mats) throw std::invalid_argument("[slice] matrix index out of range");
+                m_shape = Shape{p.rows, p.cols, span};
+                m_data = parent.m_data + from * p.perMatrix();
+                m_table = parent.m_table ? parent.m_table + from : nullptr;
+                break;
+            case 1:
+                if (to >= p.cols) throw std::invalid_argument("[slice] column index out of range");
+                m_shape = Shape{p.rows, span, 1};
+                m_data = parent.m_data + from * p.rows;
+                break;
+            case 0:
+                if (to >= p.count()) throw std::invalid_argument("[slice] element index out of range");
+                m_shape = Shape{span, 1, 1};
+                m_data = parent.m_data + from;
+                break;
+            default:
+                throw std::invalid_argument("[slice] axis must be 0, 1 or 2");
         }
-        m_d_data = other.m_d_data + offset;
-        m_doDestroyData = false;
-        m_doDestroyPtrMatrices = false;
-        m_idxStream = other.m_idxStream;
     }
 
-    T *raw() const { return m_d_data; }
-    T **ptrMatrices() const { return m_d_ptrMatrices; }
-    size_t numRows() const { return m_numRows; }
-    size_t numCols() const { return m_numCols; }
-    size_t numMats() const { return m_numMats; }
-    size_t numEl() const { return m_numRows * m_numCols * m_numMats; }
+    T *raw() const { return m_data; }
+    T **ptrMatrices() const { return m_table; }
+    size_t numRows() const { return m_shape.rows; }
+    size_t numCols() const { return m_shape.cols; }
+    size_t numMats() const { return m_shape.mats; }
+    size_t numEl() const { return m_shape.count(); }
 
-    /** Synchronous upload (tensor.cuh:1128-1145). */
+    /** Blocking upload of numEl() values in column- or row-major order (tensor.cuh:1128-1145). */
     bool upload(const std::vector<T> &vec, StorageMode mode = StorageMode::defaultMajor) {
-        const size_t size = vec.size();
-        const size_t thisSize = numEl();
-        if (size != thisSize) throw std::invalid_argument("[upload] vec has wrong size");
-        Session::getInstance().synchronizeAllStreams();
+        if (vec.size() != numEl()) throw std::invalid_argument("[upload] vec has wrong size");
         if (mode == StorageMode::rowMajor) {
-            std::vector<T> vecCm(thisSize);
-            rm2cm(vec, vecCm);
-            gpuErrChk(gputils_memcpy(m_d_data, vecCm.data(), size * sizeof(T), 0));
+            const std::vector<T> columnMajorData = gputils_detail::rowToColumnMajor(vec, m_shape);
+            gputils_detail::blockingCopy(m_data, columnMajorData.data(), numEl() * sizeof(T), 0);
         } else {
-            gpuErrChk(gputils_memcpy(m_d_data, vec.data(), size * sizeof(T), 0));
+            gputils_detail::blockingCopy(m_data, vec.data(), numEl() * sizeof(T), 0);
         }
         return true;
     }
 
-    /** Synchronous download of all elements (tensor.cuh:1147-1154). */
+    /** Blocking download of all elements, column-major (tensor.cuh:1147-1154). */
     void download(std::vector<T> &vec) const {
         vec.resize(numEl());
-        Session::getInstance().synchronizeAllStreams();
-        gpuErrChk(gputils_memcpy(vec.data(), m_d_data, numEl() * sizeof(T), 1));
+        gputils_detail::blockingCopy(vec.data(), m_data, numEl() * sizeof(T), 1);
     }
 
     /** Device-to-device copy into a tensor with at least numEl() elements (tensor.cuh:1199-1208). */
     void deviceCopyTo(DTensor<T> &elsewhere) const {
         if (elsewhere.numEl() < numEl()) throw std::invalid_argument("[deviceCopyTo] tensor does not fit into destination");
-        Session::getInstance().synchronizeAllStreams();
-        gpuErrChk(gputils_memcpy(elsewhere.raw(), m_d_data, numEl() * sizeof(T), 2));
+        gputils_detail::blockingCopy(elsewhere.raw(), m_data, numEl() * sizeof(T), 2);
     }
 
     /** Rows rowsFrom..rowsTo (inclusive) of matrix matIdx as a new tensor (tensor.cuh:1396-1424). */
     DTensor<T> getRows(size_t rowsFrom, size_t rowsTo, size_t matIdx) const {
+        if (rowsFrom > rowsTo || rowsTo >= m_shape.rows || matIdx >= m_shape.mats)
+            throw std::invalid_argument("[getRows] index out of range");
         const size_t len = rowsTo - rowsFrom + 1;
-        DTensor<T> rowsOnly(len, m_numCols, 1);
-        const T *src = m_d_data + matIdx * m_numRows * m_numCols + rowsFrom;
+        DTensor<T> rowsOnly(len, m_shape.cols, 1);
+        const T *src = m_data + matIdx * m_shape.perMatrix() + rowsFrom;
         // rows of a column-major matrix: one contiguous copy per column
         Session::getInstance().synchronizeAllStreams();
-        for (size_t c = 0; c < m_numCols; c++)
-            gpuErrChk(gputils_memcpy(rowsOnly.raw() + c * len, src + c * m_numRows, len * sizeof(T), 2));
+        for (size_t c = 0; c < m_shape.cols; c++)
+            gpuErrChk(gputils_memcpy(rowsOnly.raw() + c * len, src + c * m_shape.rows, len * sizeof(T), 2));
         return rowsOnly;
     }
 
     /** Batched transpose (tensor.cuh:1167-1197, one launch for all matrices). */
     DTensor<T> tr() const {
         gputils_detail::require_fp<T>();
-        DTensor<T> transposes(m_numCols, m_numRows, m_numMats);
+        DTensor<T> out(m_shape.cols, m_shape.rows, m_shape.mats);
+        const int64_t m = (int64_t)m_shape.rows, n = (int64_t)m_shape.cols, per = m * n;
         if constexpr (std::is_same_v<T, double>)
-            gpuErrChk(gputils_dtranspose_batched(m_numRows, m_numCols, raw(), m_numRows, m_numRows * m_numCols,
-                                                 transposes.raw(), m_numCols, m_numRows * m_numCols, m_numMats, stream()));
+            gpuErrChk(gputils_dtranspose_batched(m, n, raw(), m, per, out.raw(), n, per, (int64_t)m_shape.mats, stream()));
         if constexpr (std::is_same_v<T, float>)
-            gpuErrChk(gputils_stranspose_batched(m_numRows, m_numCols, raw(), m_numRows, m_numRows * m_numCols,
-                                                 transposes.raw(), m_numCols, m_numRows * m_numCols, m_numMats, stream()));
-        return transposes;
+            gpuErrChk(gputils_stranspose_batched(m, n, raw(), m, per, out.raw(), n, per, (int64_t)m_shape.mats, stream()));
+        return out;
     }
 
     /** Frobenius dot product (tensor.cuh:968-992). */
@@ -461,45 +541,39 @@ public:
     void leastSquaresBatched(DTensor &b);
 
     /** C_i <- beta C_i + alpha A_i B_i (tensor.cuh:1286-1338). */
-    void addAB(const DTensor<T> &A, const DTensor<T> &B, T alpha = 1, T beta = 0);
-
-    /**
-     * Reshape keeping the element count (tensor.cuh:828-861).  The pointer
-     * array is reallocated when the matrix count outgrows it; unlike the
-     * reference, the byte counter always subtracts exactly what was
-     * allocated.
-     */
-    void reshape(size_t newNumRows, size_t newNumCols, size_t newNumMats = 1) {
-        if (m_numRows == newNumRows && m_numCols == newNumCols && m_numMats == newNumMats) return;
-        const size_t newNumElements = newNumRows * newNumCols * newNumMats;
-        if (numEl() != newNumElements) {
-            char errMessage[256];
-            snprintf(errMessage, sizeof(errMessage),
-                     "DTensor[%zu x %zu x %zu] with %zu elements cannot be reshaped into DTensor[%zu x %zu x %zu] (%zu elements)",
-                     numRows(), numCols(), numMats(), numEl(), newNumRows, newNumCols, newNumMats, newNumElements);
-            throw std::invalid_argument(errMessage);
-        }
-        if (newNumMats > m_numMats && newNumMats > m_ptrCapacity) {
-            if (m_d_ptrMatrices && m_doDestroyPtrMatrices) {
-                gpuErrChk(gputils_free(m_d_ptrMatrices));
-                Session::getInstance().incrementAllocatedBytes(-(long long)(m_ptrCapacity * sizeof(T *)));
-                m_d_ptrMatrices = nullptr;
-                m_doDestroyPtrMatrices = false;
-                m_ptrCapacity = 0;
-            }
-            if (newNumMats > 1) {
-                gpuErrChk(Session::getInstance().allocate((void **)&m_d_ptrMatrices, newNumMats * sizeof(T *)));
-                m_doDestroyPtrMatrices = true;
-                m_ptrCapacity = newNumMats;
-            }
-        }
-        m_numRows = newNumRows;
-        m_numCols = newNumCols;
-        m_numMats = newNumMats;
-        initialisePointersToMatricesData();
+    void addAB(const DTensor<T> &A, const DTensor<T> &B, T alpha = 1, T beta = 0) {
+        addAB(A, B, alpha, beta, GemmMode::standard);
     }
 
-    /** Save as text, or binary .bt (tensor.cuh:797-825). */
+    /**
+     * As addAB above with a choice of arithmetic (no reference
+     * counterpart): GemmMode::bf16x3 (float only) runs three bfloat16 MFMA
+     * products of the split operands with FP32 accumulation.
+     */
+    void addAB(const DTensor<T> &A, const DTensor<T> &B, T alpha, T beta, GemmMode mode);
+
+    /**
+     * Reinterprets the elements as (newNumRows, newNumCols, newNumMats)
+     * (tensor.cuh:828-861); the element count must not change.  An owned
+     * pointer table is refilled (and reallocated only when it is too small);
+     * a slice's borrowed table is replaced by an owned one unless the matrix
+     * size is unchanged and the matrix count does not grow.  The byte
+     * counter always changes by exactly what is allocated or freed.
+     */
+    void reshape(size_t newNumRows, size_t newNumCols, size_t newNumMats = 1) {
+        const Shape next{newNumRows, newNumCols, newNumMats};
+        if (next == m_shape) return;
+        if (next.count() != m_shape.count())
+            throw std::invalid_argument("[reshape] cannot view the " + std::to_string(m_shape.count()) +
+                                        " elements of DTensor" + m_shape.str() + " as DTensor" + next.str() +
+                                        " (" + std::to_string(next.count()) + " elements)");
+        const bool borrowed = m_table != nullptr && m_tableStorage.empty();
+        const bool borrowedStillValid = borrowed && next.perMatrix() == m_shape.perMatrix() && next.mats <= m_shape.mats;
+        m_shape = next;
+        rebuildPointerTable(borrowedStillValid);
+    }
+
+    /** Save as text, or as binary when the name ends in .bt (tensor.cuh:797-825). */
     void saveToFile(std::string pathToFile);
 
     /**
@@ -515,69 +589,78 @@ public:
 
     /* ------------- operators ------------- */
 
-    /** Shallow alias (tensor.cuh:1219-1228). */
+    /**
+     * Alias: this tensor views other's elements and pointer table
+     * (tensor.cuh:1219-1228).  Storage this tensor owned is released first.
+     */
     DTensor &operator=(const DTensor &other) {
-        m_numMats = other.m_numMats;
-        m_numRows = other.m_numRows;
-        m_numCols = other.m_numCols;
-        m_doDestroyData = false;
-        m_d_data = other.m_d_data;
+        if (this == &other) return *this;
+        // a view into this tensor's own storage keeps that storage alive
+        if (!m_elements.holds(other.m_data)) m_elements.release();
+        if (!m_tableStorage.holds(other.m_table)) m_tableStorage.release();
+        m_shape = other.m_shape;
+        m_data = other.m_data;
+        m_table = other.m_table;
         m_idxStream = other.m_idxStream;
         return *this;
     }
 
-    /** Element (i,j,k) by a synchronous single-element copy (tensor.cuh:1278-1284). */
+    /** Takes over a temporary's storage (an alias of it would dangle). */
+    DTensor &operator=(DTensor &&other) noexcept {
+        if (this == &other) return *this;
+        if (other.m_elements.empty()) return *this = static_cast<const DTensor &>(other);  // a view: alias it
+        m_elements = std::move(other.m_elements);
+        m_tableStorage = std::move(other.m_tableStorage);
+        m_shape = std::exchange(other.m_shape, Shape{});
+        m_data = std::exchange(other.m_data, nullptr);
+        m_table = std::exchange(other.m_table, nullptr);
+        m_idxStream = other.m_idxStream;
+        return *this;
+    }
+
+    /** Element (i,j,k) by a blocking single-element copy (tensor.cuh:1278-1284). */
     T operator()(size_t i, size_t j = 0, size_t k = 0) const {
-        T hostDst;
-        const size_t offset = i + m_numRows * (j + m_numCols * k);
-        Session::getInstance().synchronizeAllStreams();
-        gpuErrChk(gputils_memcpy(&hostDst, m_d_data + offset, sizeof(T), 1));
-        return hostDst;
+        const size_t at = i + m_shape.rows * (j + m_shape.cols * k);
+        if (at >= numEl()) throw std::out_of_range("[DTensor] element index out of range");
+        T value;
+        gputils_detail::blockingCopy(&value, m_data + at, sizeof(T), 1);
+        return value;
     }
 
     DTensor &operator*=(T scalar) {
         gputils_detail::require_fp<T>();
-        if constexpr (std::is_same_v<T, double>) gpuErrChk(gputils_dscal(numEl(), scalar, m_d_data, stream()));
-        if constexpr (std::is_same_v<T, float>) gpuErrChk(gputils_sscal(numEl(), scalar, m_d_data, stream()));
+        if constexpr (std::is_same_v<T, double>) gpuErrChk(gputils_dscal(numEl(), scalar, m_data, stream()));
+        if constexpr (std::is_same_v<T, float>) gpuErrChk(gputils_sscal(numEl(), scalar, m_data, stream()));
         return *this;
     }
 
     DTensor &operator+=(const DTensor &rhs) {
         gputils_detail::require_fp<T>();
-        if constexpr (std::is_same_v<T, double>) gpuErrChk(gputils_daxpy(numEl(), 1.0, rhs.m_d_data, m_d_data, stream()));
-        if constexpr (std::is_same_v<T, float>) gpuErrChk(gputils_saxpy(numEl(), 1.0f, rhs.m_d_data, m_d_data, stream()));
+        axpyFrom(T(1), rhs);
         return *this;
     }
 
     DTensor &operator-=(const DTensor &rhs) {
         gputils_detail::require_fp<T>();
-        if constexpr (std::is_same_v<T, double>) gpuErrChk(gputils_daxpy(numEl(), -1.0, rhs.m_d_data, m_d_data, stream()));
-        if constexpr (std::is_same_v<T, float>) gpuErrChk(gputils_saxpy(numEl(), -1.0f, rhs.m_d_data, m_d_data, stream()));
+        axpyFrom(T(-1), rhs);
         return *this;
     }
 
-    friend DTensor operator+(DTensor &first, const DTensor &second) {
-        DTensor result(first);
-        result += second;
-        return result;
-    }
+    friend DTensor operator+(DTensor &first, const DTensor &second) { return combined(first, second, T(1)); }
 
-    friend DTensor operator-(DTensor &first, const DTensor &second) {
-        DTensor result(first);
-        result -= second;
-        return result;
-    }
+    friend DTensor operator-(DTensor &first, const DTensor &second) { return combined(first, second, T(-1)); }
 
+    /** New tensor A_i B_i with B's matrix count (tensor.cuh:645-650). */
     friend DTensor<T> operator*(DTensor &A, DTensor &B) {
-        DTensor<T> result(A.m_numRows, B.m_numCols, B.m_numMats);
-        result.addAB(A, B);
-        return result;
+        DTensor<T> product(A.numRows(), B.numCols(), B.numMats());
+        product.addAB(A, B);
+        return product;
     }
 
     friend DTensor<T> operator*(T a, DTensor &B) {
-        DTensor<T> result(B);
-        result *= a;
-        return result;
+        DTensor<T> scaled(B);
+        scaled *= a;
+        return scaled;
     }
 
     friend std::ostream &operator<<(std::ostream &out, const DTensor<T> &data) { return data.print(out); }
@@ -590,7 +673,7 @@ template <typename S>
 S DTensor<T>::reduceToHost(int (*fn)(int64_t, const S *, S *, void *, void *)) const {
     Session &s = Session::getInstance();
     S *slot = static_cast<S *>(s.scalarSlot(m_idxStream));
-    gpuErrChk(fn((int64_t)numEl(), m_d_data, slot, s.reduceWorkspace(m_idxStream), stream()));
+    gpuErrChk(fn((int64_t)numEl(), m_data, slot, s.reduceWorkspace(m_idxStream), stream()));
     S host;
     gpuErrChk(gputils_memcpy_async(&host, slot, sizeof(S), 1, stream()));
     gpuErrChk(gputils_stream_synchronize(stream()));
@@ -619,14 +702,13 @@ struct Reducers<float> {
 template <typename T>
 T DTensor<T>::dotF(const DTensor &other) {
     gputils_detail::require_fp<T>();
-    if (m_numRows != other.m_numRows || m_numCols != other.m_numCols || m_numMats != other.m_numMats)
-        throw std::invalid_argument("[dotF] incompatible dimensions");
+    if (!(m_shape == other.m_shape)) throw std::invalid_argument("[dotF] incompatible dimensions");
     Session &s = Session::getInstance();
     T *slot = static_cast<T *>(s.scalarSlot(m_idxStream));
     if constexpr (std::is_same_v<T, double>)
-        gpuErrChk(gputils_ddot(numEl(), m_d_data, other.m_d_data, slot, s.reduceWorkspace(m_idxStream), stream()));
+        gpuErrChk(gputils_ddot(numEl(), m_data, other.m_data, slot, s.reduceWorkspace(m_idxStream), stream()));
     if constexpr (std::is_same_v<T, float>)
-        gpuErrChk(gputils_sdot(numEl(), m_d_data, other.m_d_data, slot, s.reduceWorkspace(m_idxStream), stream()));
+        gpuErrChk(gputils_sdot(numEl(), m_data, other.m_data, slot, s.reduceWorkspace(m_idxStream), stream()));
     T host;
     gpuErrChk(gputils_memcpy_async(&host, slot, sizeof(T), 1, stream()));
     gpuErrChk(gputils_stream_synchronize(stream()));
@@ -655,16 +737,21 @@ T DTensor<T>::minAbs() const {
 }
 
 template <typename T>
-void DTensor<T>::addAB(const DTensor<T> &A, const DTensor<T> &B, T alpha, T beta) {
+void DTensor<T>::addAB(const DTensor<T> &A, const DTensor<T> &B, T alpha, T beta, GemmMode mode) {
     gputils_detail::require_fp<T>();
-    const size_t nMat = A.numMats(), nRA = A.numRows(), nCA = A.numCols(), nCB = B.numCols();
+    const int64_t m = (int64_t)A.numRows(), inner = (int64_t)A.numCols(), n = (int64_t)B.numCols();
+    const int64_t count = (int64_t)A.numMats();
     /* like the reference: sizes from A and B, k from A, no further checks */
-    if constexpr (std::is_same_v<T, double>)
-        gpuErrChk(gputils_dgemm_strided_batched(nRA, nCB, nCA, alpha, A.raw(), nRA, nRA * nCA, B.raw(), nCA, nCA * nCB,
-                                                beta, raw(), nRA, nRA * nCB, nMat, stream()));
-    if constexpr (std::is_same_v<T, float>)
-        gpuErrChk(gputils_sgemm_strided_batched(nRA, nCB, nCA, alpha, A.raw(), nRA, nRA * nCA, B.raw(), nCA, nCA * nCB,
-                                                beta, raw(), nRA, nRA * nCB, nMat, GPUTILS_GEMM_FP32, stream()));
+    if constexpr (std::is_same_v<T, double>) {
+        if (mode != GemmMode::standard) throw std::invalid_argument("[addAB] GemmMode::bf16x3 needs T = float");
+        gpuErrChk(gputils_dgemm_strided_batched(m, n, inner, alpha, A.raw(), m, m * inner, B.raw(), inner, inner * n,
+                                                beta, raw(), m, m * n, count, stream()));
+    }
+    if constexpr (std::is_same_v<T, float>) {
+        const int abiMode = mode == GemmMode::bf16x3 ? GPUTILS_GEMM_BF16X3 : GPUTILS_GEMM_FP32;
+        gpuErrChk(gputils_sgemm_strided_batched(m, n, inner, alpha, A.raw(), m, m * inner, B.raw(), inner, inner * n,
+                                                beta, raw(), m, m * n, count, abiMode, stream()));
+    }
 }
 
 /**
@@ -675,140 +762,166 @@ void DTensor<T>::addAB(const DTensor<T> &A, const DTensor<T> &B, T alpha, T beta
 template <typename T>
 void DTensor<T>::leastSquaresBatched(DTensor &B) {
     gputils_detail::require_fp<T>();
-    const size_t batchSize = numMats(), nColsB = B.numCols();
-    if (B.numRows() != m_numRows)
+    if (B.numRows() != numRows())
         throw std::invalid_argument("[Least squares batched] rhs rows does not equal lhs rows");
-    if (nColsB != 1) throw std::invalid_argument("[Least squares batched] rhs are not vectors");
-    if (B.numMats() != batchSize)
+    if (B.numCols() != 1) throw std::invalid_argument("[Least squares batched] rhs are not vectors");
+    if (B.numMats() != numMats())
         throw std::invalid_argument("[Least squares batched] rhs numMats does not equal lhs numMats");
-    if (m_numCols > m_numRows)
+    if (numCols() > numRows())
         throw std::invalid_argument("[Least squares batched] supports square or tall matrices only");
-    const int64_t m = (int64_t)m_numRows, n = (int64_t)m_numCols;
+    const int64_t m = (int64_t)numRows(), n = (int64_t)numCols(), count = (int64_t)numMats();
     if constexpr (std::is_same_v<T, double>)
-        gpuErrChk(gputils_dgels_batched(m, n, 1, raw(), m, m * n, B.raw(), m, m, nullptr, (int64_t)batchSize, stream()));
+        gpuErrChk(gputils_dgels_batched(m, n, 1, raw(), m, m * n, B.raw(), m, m, nullptr, count, stream()));
     if constexpr (std::is_same_v<T, float>)
-        gpuErrChk(gputils_sgels_batched(m, n, 1, raw(), m, m * n, B.raw(), m, m, nullptr, (int64_t)batchSize, stream()));
+        gpuErrChk(gputils_sgels_batched(m, n, 1, raw(), m, m * n, B.raw(), m, m, nullptr, count, stream()));
 }
 
+/** Tensor of uniform random entries in [low, hi]; unseeded (tensor.cuh:690-702). */
 template <typename T>
 DTensor<T> DTensor<T>::createRandomTensor(size_t numRows, size_t numCols, size_t numMats, T low, T hi) {
-    if constexpr (std::is_floating_point<T>::value) {
-        auto randVec = generateRealRandomVector<T>(numRows * numCols * numMats, low, hi);
-        return DTensor<T>(randVec, numRows, numCols, numMats);
+    const size_t count = numRows * numCols * numMats;
+    if constexpr (std::is_floating_point_v<T>) {
+        return DTensor<T>(generateRealRandomVector<T>(count, low, hi), numRows, numCols, numMats);
     } else if constexpr (std::is_same_v<T, int>) {
-        auto randVec = generateIntRandomVector(numRows * numCols * numMats, low, hi);
-        return DTensor<T>(randVec, numRows, numCols, numMats);
+        return DTensor<T>(generateIntRandomVector(count, low, hi), numRows, numCols, numMats);
+    } else {
+        throw std::invalid_argument("[createRandomTensor] unsupported type T");
     }
-    throw std::invalid_argument("[createRandomTensor] unsupported type T");
 }
 
 /* --------------------------------------------------------- file I/O --- */
+/*
+ * Two on-disk forms of an (m, n, k) tensor (tensor.cuh:704-825):
+ *  - text: m, n and k on the first three lines, then the m*n*k elements in
+ *    column-major order, one per line (missing trailing values read as 0);
+ *  - binary (file name ending in ".bt"): m, n, k as three little-endian
+ *    uint64 values, then the elements as raw sizeof(T)-byte values in
+ *    column-major order.
+ */
+namespace gputils_detail {
+
+inline bool isBinaryTensorFile(const std::string &path) {
+    static const std::string ext = ".bt";
+    return path.size() >= ext.size() && path.compare(path.size() - ext.size(), ext.size(), ext) == 0;
+}
+
+/** One text field as an arithmetic value of type V. */
+template <typename V>
+V parseTextValue(const std::string &field) {
+    static_assert(std::is_arithmetic_v<V>, "tensor files hold arithmetic element types only");
+    const char *begin = field.c_str();
+    char *end = nullptr;
+    errno = 0;
+    V value;
+    if constexpr (std::is_floating_point_v<V>) {
+        value = static_cast<V>(std::strtold(begin, &end));
+    } else if constexpr (std::is_signed_v<V>) {
+        value = static_cast<V>(std::strtoll(begin, &end, 10));
+    } else {
+        value = static_cast<V>(std::strtoull(begin, &end, 10));
+    }
+    if (end == begin) throw std::invalid_argument("[parseFromFile] not a number: '" + field + "'");
+    return value;
+}
 
 template <typename T>
-struct data_t {
-    size_t numRows;
-    size_t numCols;
-    size_t numMats;
-    std::vector<T> data;
+struct TensorFileContents {
+    Shape shape;
+    std::vector<T> values;
 };
 
-/** Text format: rows, cols, mats, then one value per line (tensor.cuh:713-760). */
 template <typename T>
-data_t<T> vectorFromTextFile(std::string path_to_file) {
-    data_t<T> d;
-    std::ifstream file(path_to_file, std::ios::in);
-    if (!file.is_open()) throw std::invalid_argument("[vectorFromTextFile] the file does not exist");
+TensorFileContents<T> readTextTensor(const std::string &path) {
+    std::ifstream in(path);
+    if (!in) throw std::invalid_argument("[parseFromFile] cannot open '" + path + "'");
+    size_t dims[3];
     std::string line;
-    getline(file, line);
-    d.numRows = atoi(line.c_str());
-    getline(file, line);
-    d.numCols = atoi(line.c_str());
-    getline(file, line);
-    d.numMats = atoi(line.c_str());
-    const size_t numElements = d.numRows * d.numCols * d.numMats;
-    std::vector<T> v(numElements);
-    size_t i = 0;
-    while (i < numElements && getline(file, line)) {
-        if constexpr (std::is_same_v<T, int>) v[i] = atoi(line.c_str());
-        else if constexpr (std::is_same_v<T, double>) v[i] = std::stod(line);
-        else if constexpr (std::is_same_v<T, float>) v[i] = std::stof(line);
-        else if constexpr (std::is_same_v<T, long double>) v[i] = std::stold(line);
-        else if constexpr (std::is_same_v<T, long>) v[i] = std::stol(line);
-        else if constexpr (std::is_same_v<T, long long>) v[i] = std::stoll(line);
-        else if constexpr (std::is_same_v<T, unsigned long>) v[i] = std::stoul(line);
-        else if constexpr (std::is_same_v<T, unsigned long long>) v[i] = std::stoull(line);
-        else throw std::invalid_argument("data type not supported");
-        ++i;
+    for (size_t &d : dims) {
+        if (!std::getline(in, line)) throw std::invalid_argument("[parseFromFile] '" + path + "' has no header");
+        d = parseTextValue<unsigned long long>(line);
     }
-    d.data = v;
-    return d;
+    TensorFileContents<T> f{Shape{dims[0], dims[1], dims[2]}, {}};
+    f.values.assign(f.shape.count(), T(0));
+    for (T &v : f.values) {
+        if (!std::getline(in, line)) break;
+        v = parseTextValue<T>(line);
+    }
+    return f;
 }
 
-/** Binary .bt: three little-endian uint64 (rows, cols, mats) then the data (tensor.cuh:762-782). */
 template <typename T>
-data_t<T> vectorFromBinaryFile(std::string path_to_file) {
-    data_t<T> d;
-    std::ifstream in(path_to_file, std::ios::binary);
-    if (!in.is_open()) throw std::invalid_argument("[vectorFromBinaryFile] the file does not exist");
-    uint64_t hdr[3] = {0, 0, 0};
-    in.read(reinterpret_cast<char *>(hdr), sizeof(hdr));
-    d.numRows = hdr[0];
-    d.numCols = hdr[1];
-    d.numMats = hdr[2];
-    const uint64_t numElements = hdr[0] * hdr[1] * hdr[2];
-    d.data.resize(numElements);
-    in.read(reinterpret_cast<char *>(d.data.data()), numElements * sizeof(T));
-    return d;
+TensorFileContents<T> readBinaryTensor(const std::string &path) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw std::invalid_argument("[parseFromFile] cannot open '" + path + "'");
+    uint64_t dims[3];
+    if (!in.read(reinterpret_cast<char *>(dims), sizeof dims))
+        throw std::invalid_argument("[parseFromFile] '" + path + "' is shorter than its header");
+    TensorFileContents<T> f{Shape{dims[0], dims[1], dims[2]}, {}};
+    f.values.resize(f.shape.count());
+    const std::streamsize payload = static_cast<std::streamsize>(f.values.size() * sizeof(T));
+    in.read(reinterpret_cast<char *>(f.values.data()), payload);
+    if (in.gcount() != payload) throw std::invalid_argument("[parseFromFile] '" + path + "' is truncated");
+    return f;
 }
+
+template <typename T>
+void writeTensorFile(const std::string &path, const Shape &shape, const std::vector<T> &values) {
+    if (isBinaryTensorFile(path)) {
+        std::ofstream out(path, std::ios::binary);
+        const uint64_t dims[3] = {shape.rows, shape.cols, shape.mats};
+        out.write(reinterpret_cast<const char *>(dims), sizeof dims);
+        out.write(reinterpret_cast<const char *>(values.data()), (std::streamsize)(values.size() * sizeof(T)));
+        if (!out) throw std::runtime_error("[saveToFile] cannot write '" + path + "'");
+        return;
+    }
+    std::ostringstream text;
+    if constexpr (std::is_floating_point_v<T>) text << std::setprecision(std::numeric_limits<T>::max_digits10);
+    text << shape.rows << '\n' << shape.cols << '\n' << shape.mats << '\n';
+    for (const T &v : values) text << v << '\n';
+    std::ofstream out(path);
+    out << text.str();
+    if (!out) throw std::runtime_error("[saveToFile] cannot write '" + path + "'");
+}
+}  // namespace gputils_detail
 
 template <typename T>
 DTensor<T> DTensor<T>::parseFromFile(std::string path_to_file, StorageMode mode) {
-    const bool binary = path_to_file.size() >= 3 && path_to_file.substr(path_to_file.size() - 3) == ".bt";
-    data_t<T> parsed = binary ? vectorFromBinaryFile<T>(path_to_file) : vectorFromTextFile<T>(path_to_file);
-    return DTensor<T>(parsed.data, parsed.numRows, parsed.numCols, parsed.numMats, mode);
+    auto contents = gputils_detail::isBinaryTensorFile(path_to_file) ? gputils_detail::readBinaryTensor<T>(path_to_file)
+                                                                     : gputils_detail::readTextTensor<T>(path_to_file);
+    const auto &s = contents.shape;
+    return DTensor<T>(contents.values, s.rows, s.cols, s.mats, mode);
+}
+
+template <typename T>
+void DTensor<T>::saveToFile(std::string pathToFile) {
+    std::vector<T> host;
+    download(host);
+    gputils_detail::writeTensorFile(pathToFile, m_shape, host);
 }
 
 template <typename T>
 void DTensor<T>::applyRightGivensRotation(size_t i, size_t j, const T *c, const T *minus_s) {
     gputils_detail::require_fp<T>();
-    if (m_numMats > 1) throw std::invalid_argument("[applyRightGivensRotation] tensors (nMat>1) not supported");
-    if (i >= m_numCols || j >= m_numCols) throw std::invalid_argument("[applyRightGivensRotation] invalid column index");
-    T *col_i = m_d_data + i * m_numRows;
-    T *col_j = m_d_data + j * m_numRows;
+    if (numMats() > 1) throw std::invalid_argument("[applyRightGivensRotation] tensors (nMat>1) not supported");
+    if (i >= numCols() || j >= numCols()) throw std::invalid_argument("[applyRightGivensRotation] invalid column index");
+    T *col_i = m_data + i * numRows();
+    T *col_j = m_data + j * numRows();
     if constexpr (std::is_same_v<T, double>)
-        gpuErrChk(gputils_drot((int64_t)m_numRows, col_i, 1, col_j, 1, *c, *minus_s, stream()));
+        gpuErrChk(gputils_drot((int64_t)numRows(), col_i, 1, col_j, 1, *c, *minus_s, stream()));
     if constexpr (std::is_same_v<T, float>)
-        gpuErrChk(gputils_srot((int64_t)m_numRows, col_i, 1, col_j, 1, *c, *minus_s, stream()));
+        gpuErrChk(gputils_srot((int64_t)numRows(), col_i, 1, col_j, 1, *c, *minus_s, stream()));
 }
 
 template <typename T>
 void DTensor<T>::applyLeftGivensRotation(size_t i, size_t j, const T *c, const T *minus_s) {
     gputils_detail::require_fp<T>();
-    if (m_numMats > 1) throw std::invalid_argument("[applyLeftGivensRotation] tensors (nMat>1) not supported");
-    if (i >= m_numRows || j >= m_numRows) throw std::invalid_argument("[applyLeftGivensRotation] invalid row index");
-    const int64_t ld = (int64_t)m_numRows;
+    if (numMats() > 1) throw std::invalid_argument("[applyLeftGivensRotation] tensors (nMat>1) not supported");
+    if (i >= numRows() || j >= numRows()) throw std::invalid_argument("[applyLeftGivensRotation] invalid row index");
+    const int64_t ld = (int64_t)numRows();
     if constexpr (std::is_same_v<T, double>)
-        gpuErrChk(gputils_drot((int64_t)m_numCols, m_d_data + i, ld, m_d_data + j, ld, *c, *minus_s, stream()));
+        gpuErrChk(gputils_drot((int64_t)numCols(), m_data + i, ld, m_data + j, ld, *c, *minus_s, stream()));
     if constexpr (std::is_same_v<T, float>)
-        gpuErrChk(gputils_srot((int64_t)m_numCols, m_d_data + i, ld, m_d_data + j, ld, *c, *minus_s, stream()));
-}
-
-template <typename T>
-void DTensor<T>::saveToFile(std::string pathToFile) {
-    std::vector<T> myData(numEl());
-    download(myData);
-    const bool binary = pathToFile.size() >= 3 && pathToFile.substr(pathToFile.size() - 3) == ".bt";
-    if (binary) {
-        const uint64_t hdr[3] = {(uint64_t)numRows(), (uint64_t)numCols(), (uint64_t)numMats()};
-        std::ofstream out(pathToFile, std::ios::binary);
-        out.write(reinterpret_cast<const char *>(hdr), sizeof(hdr));
-        out.write(reinterpret_cast<const char *>(myData.data()), myData.size() * sizeof(T));
-    } else {
-        std::ofstream file(pathToFile);
-        file << numRows() << std::endl << numCols() << std::endl << numMats() << std::endl;
-        if constexpr (std::is_floating_point<T>::value) file << std::setprecision(std::numeric_limits<T>::max_digits10);
-        for (const T &el : myData) file << el << std::endl;
-    }
+        gpuErrChk(gputils_srot((int64_t)numCols(), m_data + i, ld, m_data + j, ld, *c, *minus_s, stream()));
 }
 
 /* ------------------------------------------------------------ status --- */
@@ -878,6 +991,56 @@ public:
             gpuErrChk(gputils_spotrs_batched(n, nrhs, m_matrix->raw(), n, n * n, b.raw(), n, n * nrhs, m_info->raw(),
                                              m_numMats, m_matrix->stream()));
     }
+};
+
+/**
+ * Mixed-precision batched SPD solve (no reference counterpart; it sits
+ * beside CholeskyBatchFactoriser, tensor.cuh:2098-2197).  Each A_i
+ * (FP64, lower triangle used, read only) is factorised in FP32 on the GPU;
+ * solves are refined in FP64 until every right-hand side meets
+ *   ||b - A x||_inf <= tol (||A||_inf ||x||_inf + ||b||_inf)
+ * (LAPACK DSPOSV's test).  Systems that do not converge in maxIter steps,
+ * or whose FP32 factorisation fails, are solved again in FP64: then
+ * iterations()(0,0,i) = -1 and info()(0,0,i) is the FP64 status (0, or
+ * the order of the first non-positive leading minor).  A must outlive this
+ * object.  n <= 128 runs the fused mixed kernel; larger n uses the FP64
+ * kernels directly (iterations = -1).
+ */
+class MixedCholeskyBatchSolver : public IStatus {
+private:
+    DTensor<double> *m_matrix;
+    DTensor<int> m_iters;
+    gputils_detail::DeviceBuffer m_work;  ///< FP64 copy of A, only for n beyond the fused kernel
+    double m_tol;
+    int m_maxIter;
+
+public:
+    MixedCholeskyBatchSolver(DTensor<double> &A, double tol = 1e-12, int maxIter = 30)
+        : IStatus(A.numMats()), m_matrix(&A), m_iters(1, 1, A.numMats(), true), m_tol(tol), m_maxIter(maxIter) {
+        if (A.numRows() != A.numCols()) throw std::invalid_argument("[MixedCholesky] A must be square");
+        if (!(tol > 0)) throw std::invalid_argument("[MixedCholesky] tol must be positive");
+        if (maxIter < 0) throw std::invalid_argument("[MixedCholesky] maxIter must be non-negative");
+        const int64_t wb = gputils_dposv_mixed_workspace_bytes((int64_t)A.numRows(), (int64_t)A.numMats());
+        if (wb < 0) throw std::invalid_argument("[MixedCholesky] unsupported size");
+        m_work = gputils_detail::DeviceBuffer((size_t)wb, false);
+    }
+
+    /** x_i <- A_i^{-1} b_i for every column of b_i; b is read only, x has b's shape. */
+    void solve(const DTensor<double> &b, DTensor<double> &x) {
+        const size_t n = m_matrix->numRows(), k = m_matrix->numMats();
+        if (b.numRows() != n || b.numMats() != k || b.numCols() < 1)
+            throw std::invalid_argument("[MixedCholesky] A and b incompatible");
+        if (x.numRows() != n || x.numCols() != b.numCols() || x.numMats() != k)
+            throw std::invalid_argument("[MixedCholesky] x must have the shape of b");
+        if (x.raw() == b.raw()) throw std::invalid_argument("[MixedCholesky] x must not alias b");
+        const int64_t nn = (int64_t)n, nrhs = (int64_t)b.numCols();
+        gpuErrChk(gputils_dposv_mixed_batched(nn, nrhs, m_matrix->raw(), nn, nn * nn, b.raw(), nn, nn * nrhs, x.raw(), nn,
+                                              nn * nrhs, m_tol, m_maxIter, m_iters.raw(), m_info->raw(), m_work.get(),
+                                              (int64_t)k, m_matrix->stream()));
+    }
+
+    /** Refinement steps used per system (-1: solved by the FP64 fallback). */
+    DTensor<int> &iterations() { return m_iters; }
 };
 
 /**
@@ -990,15 +1153,81 @@ public:
     }
 };
 
+/**
+ * Least squares over many systems of uneven sizes in one plan (no
+ * reference counterpart: leastSquaresBatched, tensor.cuh:1340-1394, needs
+ * one size for all).  System s is rows[s] x cols[s] (1 <= cols <= rows
+ * <= 512, cols <= 64), stored column-major at element offsetA(s) of a flat
+ * FP64 tensor A, with its right-hand side at offsetB(s) of a flat tensor b;
+ * the default layout packs the systems back to back.  solve() overwrites
+ * the first cols[s] entries of each right-hand side with the solution of
+ * an FP32 Householder QR (FP64 norms) refined refineSteps times in FP64;
+ * A is read only.  tune() times the launch shapes of every size bucket on
+ * the caller's data and keeps the fastest (plan-time work: it allocates
+ * and synchronises).
+ */
+class RaggedLeastSquares {
+private:
+    gputils_gels_plan_t m_plan = nullptr;
+    std::vector<int64_t> m_offA, m_offB;
+    size_t m_elementsA = 0, m_elementsB = 0;
+
+    void checkData(const DTensor<double> &A, const DTensor<double> &b) const {
+        if (A.numEl() < m_elementsA) throw std::invalid_argument("[RaggedLeastSquares] A holds too few elements");
+        if (b.numEl() < m_elementsB) throw std::invalid_argument("[RaggedLeastSquares] b holds too few elements");
+    }
+
+public:
+    RaggedLeastSquares(const std::vector<int32_t> &rows, const std::vector<int32_t> &cols) {
+        if (rows.size() != cols.size()) throw std::invalid_argument("[RaggedLeastSquares] rows and cols differ in length");
+        m_offA.resize(rows.size());
+        m_offB.resize(rows.size());
+        for (size_t s = 0; s < rows.size(); s++) {
+            if (rows[s] < 1 || cols[s] < 1) throw std::invalid_argument("[RaggedLeastSquares] empty system");
+            m_offA[s] = (int64_t)m_elementsA;
+            m_offB[s] = (int64_t)m_elementsB;
+            m_elementsA += (size_t)rows[s] * (size_t)cols[s];
+            m_elementsB += (size_t)rows[s];
+        }
+        const int st = gputils_gels_ragged_plan_create(&m_plan, (int64_t)rows.size(), rows.data(), cols.data(),
+                                                       m_offA.data(), m_offB.data());
+        if (st < 0) throw std::invalid_argument(std::string("[RaggedLeastSquares] ") + gputils_last_error());
+        gpuErrChk(st);
+    }
+    ~RaggedLeastSquares() {
+        if (m_plan) gputils_gels_ragged_plan_destroy(m_plan);
+    }
+    RaggedLeastSquares(const RaggedLeastSquares &) = delete;
+    RaggedLeastSquares &operator=(const RaggedLeastSquares &) = delete;
+
+    size_t numSystems() const { return m_offA.size(); }
+    /** Elements of the packed A and b tensors this plan addresses. */
+    size_t elementsA() const { return m_elementsA; }
+    size_t elementsB() const { return m_elementsB; }
+    int64_t offsetA(size_t s) const { return m_offA.at(s); }
+    int64_t offsetB(size_t s) const { return m_offB.at(s); }
+
+    void tune(const DTensor<double> &A, const DTensor<double> &b, int refineSteps = 1) {
+        checkData(A, b);
+        gpuErrChk(gputils_gels_ragged_plan_tune(m_plan, A.raw(), b.raw(), refineSteps, A.stream()));
+    }
+
+    void solve(const DTensor<double> &A, DTensor<double> &b, int refineSteps = 2) {
+        checkData(A, b);
+        if (refineSteps < 0) throw std::invalid_argument("[RaggedLeastSquares] refineSteps must be non-negative");
+        gpuErrChk(gputils_dgels_ragged(m_plan, A.raw(), b.raw(), refineSteps, A.stream()));
+    }
+};
+
 /* --------------------------------------------------------------- SVD --- */
 
 /**
  * Batched SVD of tall matrices, A_i = U_i S_i V_i' (tensor.cuh:1500-1676).
  * All matrices are factorised by one launch (the reference loops over
- * them on the host).  At most 64 columns.  rank() counts sigma > epsilon
- * afresh on every call (the reference adds to a counter that is never
- * reset).  A is not modified by factorise(); with destroyMatrix = false a
- * private copy is factorised, as in the reference.
+ * them on the host).  rank() counts sigma > epsilon afresh on every call
+ * (the reference adds to a counter that is never reset).  A is not
+ * modified by factorise(); with destroyMatrix = false a private copy is
+ * factorised, as in the reference.
  */
 TEMPLATE_WITH_TYPE_T
 class Svd : public IStatus {
@@ -1009,19 +1238,16 @@ private:
     std::shared_ptr<DTensor<T>> m_S;
     std::shared_ptr<DTensor<T>> m_U;
     std::shared_ptr<DTensor<unsigned int>> m_rank;
-    void *m_work = nullptr;
-    size_t m_workBytes = 0;
+    gputils_detail::DeviceBuffer m_work;
     bool m_computeU = false;
-
-    void checkMatrix(DTensor<T> &tensor) const {
-        if (tensor.numRows() < tensor.numCols()) throw std::invalid_argument("[svd] your matrix is fat (no offence)");
-        if (tensor.numCols() > 64) throw std::invalid_argument("[svd] at most 64 columns are supported");
-    }
 
 public:
     Svd(DTensor<T> &mat, bool computeU = false, bool destroyMatrix = true) : IStatus(mat.numMats()) {
         gputils_detail::require_fp<T>();
-        checkMatrix(mat);
+        if (mat.numRows() < mat.numCols()) throw std::invalid_argument("[svd] your matrix is fat (no offence)");
+        const size_t m = mat.numRows(), n = mat.numCols(), k = mat.numMats();
+        const int64_t wb = gputils_gesvd_workspace_bytes((int)m, (int)n, (int64_t)k);
+        if (wb < 0) throw std::invalid_argument("[svd] unsupported matrix size");
         if (destroyMatrix) {
             m_tensor = &mat;
         } else {
@@ -1029,24 +1255,11 @@ public:
             m_tensor = m_copy.get();
         }
         m_computeU = computeU;
-        const size_t m = mat.numRows(), n = mat.numCols(), k = mat.numMats();
         m_Vtr = std::make_shared<DTensor<T>>(n, n, k);
         m_S = std::make_shared<DTensor<T>>(std::min(m, n), 1, k);
         m_rank = std::make_shared<DTensor<unsigned int>>(1, 1, k, true);
         if (computeU) m_U = std::make_shared<DTensor<T>>(m, m, k);
-        const int64_t wb = gputils_gesvd_workspace_bytes((int)m, (int)n, (int64_t)k);
-        if (wb < 0) throw std::invalid_argument("[svd] unsupported matrix size");
-        if (wb > 0) {
-            m_workBytes = (size_t)wb;
-            gpuErrChk(Session::getInstance().allocate(&m_work, m_workBytes));
-        }
-    }
-
-    ~Svd() {
-        if (m_work) {
-            gputils_free(m_work);
-            Session::getInstance().incrementAllocatedBytes(-(long long)m_workBytes);
-        }
+        m_work = gputils_detail::DeviceBuffer((size_t)wb, false);
     }
 
     Svd(const Svd &) = delete;
@@ -1058,10 +1271,10 @@ public:
         T *U = m_computeU ? m_U->raw() : nullptr;
         if constexpr (std::is_same_v<T, double>)
             gpuErrChk(gputils_dgesvd_batched(m, n, m_tensor->raw(), m, mn, m_S->raw(), n, U, m, (int64_t)m * m, m_Vtr->raw(),
-                                             n, (int64_t)n * n, m_work, m_info->raw(), nullptr, k, m_tensor->stream()));
+                                             n, (int64_t)n * n, m_work.get(), m_info->raw(), nullptr, k, m_tensor->stream()));
         if constexpr (std::is_same_v<T, float>)
             gpuErrChk(gputils_sgesvd_batched(m, n, m_tensor->raw(), m, mn, m_S->raw(), n, U, m, (int64_t)m * m, m_Vtr->raw(),
-                                             n, (int64_t)n * n, m_work, m_info->raw(), nullptr, k, m_tensor->stream()));
+                                             n, (int64_t)n * n, m_work.get(), m_info->raw(), nullptr, k, m_tensor->stream()));
     }
 
     DTensor<T> &singularValues() const { return *m_S; }
@@ -1091,7 +1304,8 @@ public:
  * Nullspace of square or fat matrices (tensor.cuh:2008-2085): N_i spans
  * ker A_i (zero-padded to n x n), P_i = N_i N_i'.  Built by one SVD launch
  * of A', one rank count and one construction launch (the reference loops
- * over the matrices on the host).
+ * over the matrices on the host).  Throws std::runtime_error if the SVD of
+ * any matrix did not converge.
  */
 TEMPLATE_WITH_TYPE_T
 class Nullspace {
@@ -1104,40 +1318,39 @@ public:
         gputils_detail::require_fp<T>();
         const size_t m = a.numRows(), n = a.numCols(), nMats = a.numMats();
         if (m > n) throw std::invalid_argument("[nullspace] I was expecting a square or fat matrix");
+        const int64_t nn = (int64_t)n * n, k = (int64_t)nMats;
+        const int64_t wb = gputils_gesvd_workspace_bytes((int)n, (int)m, k);
+        if (wb < 0) throw std::invalid_argument("[nullspace] unsupported matrix size");
         m_nullspace = std::make_unique<DTensor<T>>(n, n, nMats, true);
         m_projOp = std::make_unique<DTensor<T>>(n, n, nMats, true);
-        if (m > 64) throw std::invalid_argument("[nullspace] at most 64 rows are supported");
         DTensor<T> aTranspose = a.tr();
         // SVD of a' (n x m) without right singular vectors: the sweeps run on
         // R^T and U comes from the accumulated rotations (same subspaces)
         DTensor<T> S(m, 1, nMats), U(n, n, nMats);
         DTensor<int> info(1, 1, nMats);
         DTensor<unsigned int> rank(1, 1, nMats, true);
-        const int64_t nn = (int64_t)n * n, k = (int64_t)nMats;
-        const int64_t wb = gputils_gesvd_workspace_bytes((int)n, (int)m, k);
-        if (wb < 0) throw std::invalid_argument("[nullspace] unsupported matrix size");
-        void *work = nullptr;
-        if (wb > 0) gpuErrChk(Session::getInstance().allocate(&work, (size_t)wb));
+        gputils_detail::DeviceBuffer work((size_t)wb, false);
         void *st = aTranspose.stream();
         if constexpr (std::is_same_v<T, double>) {
             gpuErrChk(gputils_dgesvd_batched((int)n, (int)m, aTranspose.raw(), n, (int64_t)n * m, S.raw(), m, U.raw(),
-                                             n, nn, nullptr, m, (int64_t)m * m, work, info.raw(), nullptr, k, st));
+                                             n, nn, nullptr, m, (int64_t)m * m, work.get(), info.raw(), nullptr, k, st));
             gpuErrChk(gputils_dsvd_rank_batched((int)m, S.raw(), m, 1e-6, rank.raw(), k, st));
             gpuErrChk(gputils_dnullspace_build_batched((int)n, U.raw(), n, nn, rank.raw(), m_nullspace->raw(), n, nn,
                                                        m_projOp->raw(), n, nn, k, st));
         }
         if constexpr (std::is_same_v<T, float>) {
             gpuErrChk(gputils_sgesvd_batched((int)n, (int)m, aTranspose.raw(), n, (int64_t)n * m, S.raw(), m, U.raw(),
-                                             n, nn, nullptr, m, (int64_t)m * m, work, info.raw(), nullptr, k, st));
+                                             n, nn, nullptr, m, (int64_t)m * m, work.get(), info.raw(), nullptr, k, st));
             gpuErrChk(gputils_ssvd_rank_batched((int)m, S.raw(), m, 1e-6f, rank.raw(), k, st));
             gpuErrChk(gputils_snullspace_build_batched((int)n, U.raw(), n, nn, rank.raw(), m_nullspace->raw(), n, nn,
                                                        m_projOp->raw(), n, nn, k, st));
         }
         gpuErrChk(gputils_stream_synchronize(st));
-        if (work) {
-            gputils_free(work);
-            Session::getInstance().incrementAllocatedBytes(-wb);
-        }
+        std::vector<int> status;
+        info.download(status);
+        for (size_t i = 0; i < status.size(); i++)
+            if (status[i] != 0)
+                throw std::runtime_error("[nullspace] the SVD of matrix " + std::to_string(i) + " did not converge");
     }
 
     DTensor<T> const &nullspace() const { return *m_nullspace; }

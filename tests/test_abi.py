@@ -47,8 +47,8 @@ def _msg():
     ("gputils_dpotrs_batched", (4, -1, None, 4, 16, None, 4, 4, None, 1, None), -2),
     ("gputils_dpotrs_batched", (4, 1, None, 4, 16, None, 3, 4, None, 1, None), -7),
     ("gputils_dposv_batched", (129, 1, None, 129, 0, None, 129, 0, None, 1, None), -1),
-    ("gputils_dposv_mixed_batched", (8, 1, None, 8, 64, None, 8, 8, None, 8, 8, 0.0, 3, None, None, 1, None), -12),
-    ("gputils_dposv_mixed_batched", (8, 1, None, 8, 64, None, 8, 8, None, 8, 8, 1e-12, -1, None, None, 1, None), -13),
+    ("gputils_dposv_mixed_batched", (8, 1, None, 8, 64, None, 8, 8, None, 8, 8, 0.0, 3, None, None, None, 1, None), -12),
+    ("gputils_dposv_mixed_batched", (8, 1, None, 8, 64, None, 8, 8, None, 8, 8, 1e-12, -1, None, None, None, 1, None), -13),
     ("gputils_sgemm_strided_batched", (2, 2, 2, 1.0, None, 1, 4, None, 2, 4, 0.0, None, 2, 4, 1, 0, None), -6),
     ("gputils_sgemm_strided_batched", (2, 2, 2, 1.0, None, 2, 4, None, 2, 4, 0.0, None, 2, 4, 1, 7, None), -16),
     ("gputils_dgemm_strided_batched", (2, 2, 2, 1.0, None, 2, 4, None, 2, 4, 0.0, None, 1, 4, 1, None), -13),
@@ -74,8 +74,11 @@ def test_empty_problems_are_noops():
     assert L.gputils_dpotrf_batched(0, None, 1, 0, None, 5, None) == 0
     assert L.gputils_dpotrf_batched(4, None, 4, 16, None, 0, None) == 0
     assert L.gputils_sgemm_strided_batched(0, 3, 3, 1.0, None, 1, 0, None, 3, 9, 0.0, None, 1, 0, 4, 0, None) == 0
-    assert L.gputils_dposv_mixed_batched(8, 0, None, 8, 64, None, 8, 0, None, 8, 0, 1e-12, 3, None, None, 9,
-                                          None) == 0
+    assert L.gputils_dposv_mixed_batched(8, 0, None, 8, 64, None, 8, 0, None, 8, 0, 1e-12, 3, None, None, None,
+                                          9, None) == 0
+    assert L.gputils_dposv_mixed_workspace_bytes(128, 10) == 0
+    assert L.gputils_dposv_mixed_workspace_bytes(129, 10) == 129 * 129 * 8 * 10
+    assert L.gputils_dposv_mixed_workspace_bytes(-1, 10) == -1
 
 
 def test_empty_ragged_plan_shape_calls():

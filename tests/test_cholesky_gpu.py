@@ -98,6 +98,31 @@ def test_potrf_matches_oracle(oracle, n, dtype):
         assert np.all(got[b][iu] == 777.0)
 
 
+@pytest.mark.parametrize("n", [5, 16, 17, 32])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_small_potrf_reports_failing_minor(oracle, n, dtype):
+    """n <= 32 (the register-resident kernel, two or four matrices per
+    wavefront): indefinite, zero and NaN pivots give the oracle's info, and
+    their neighbours in the same wavefront factorise normally.  The batch
+    (7) is not a multiple of the matrices per wavefront."""
+    k = 7
+    A = spd_batch(k, n, seed=70 + n)
+    A[1][n // 2, n // 2] = -3.0        # fails at row n/2 + 1
+    A[2][:, :] = 0.0                   # fails at row 1
+    A[4][n - 1, n - 1] = -1e6          # fails at the last row
+    A[6][n // 3, n // 3] = np.nan      # fails at row n/3 + 1
+    t = dev(A, dtype)
+    info = api.potrf_batched(t).cpu().tolist()
+    expected = [oracle.potrf(A[b])[1] for b in range(k)]
+    assert expected == [0, n // 2 + 1, 1, 0, n, 0, n // 3 + 1]
+    assert info == expected
+    L = api.to_matrices(t).cpu().double().numpy()
+    tol = 1e-12 if dtype == torch.float64 else 1e-5
+    for b in (0, 3, 5):
+        Lo, _ = oracle.potrf(A[b])
+        assert np.max(np.abs(np.tril(L[b]) - np.tril(Lo))) <= tol * np.abs(Lo).max()
+
+
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_potrf_reports_failing_minor(oracle, dtype):
     n, k = 40, 6
@@ -196,6 +221,52 @@ def test_mixed_falls_back_to_fp64(oracle, n):
         assert backward_error(A[b:b + 1], X[b:b + 1], B[b:b + 1]) <= 1e-12
 
 
+@pytest.mark.parametrize("n", [129, 160])
+def test_mixed_beyond_the_fused_kernel_solves_in_fp64(oracle, n):
+    """n > 128: every system is solved by the FP64 kernels on a copy of A
+    (iters = -1); a non-SPD matrix reports the oracle's info; A and B stay
+    unchanged."""
+    k, nrhs = 3, 2
+    A = spd_batch(k, n, seed=400 + n)
+    A[2][7, 7] = -5.0
+    B = np.random.default_rng(n).standard_normal((k, nrhs, n))
+    tA, tB = dev(A), dev(B)
+    X, iters, info = api.posv_mixed_batched(tA, tB, tol=1e-12)
+    assert iters.cpu().tolist() == [-1] * k
+    assert info.cpu().tolist() == [0, 0, oracle.potrf(A[2])[1]] == [0, 0, 8]
+    X = X.cpu().numpy()
+    for b in range(2):
+        Lo, _ = oracle.potrf(A[b])
+        ref = oracle.potrs(Lo, B[b].T)
+        assert np.max(np.abs(X[b].T - ref)) <= 1e-12 * np.abs(ref).max()
+    assert np.array_equal(tA.cpu().numpy(), A) and np.array_equal(tB.cpu().numpy(), B)
+
+
+def test_mixed_non_finite_iterates_fall_back(oracle):
+    """The stopping test never accepts a non-finite iterate: b beyond the
+    FP32 range (its FP32 solve overflows) and a NaN right-hand side go to
+    the FP64 fallback (iters = -1); the overflow case then matches the
+    oracle.  A matrix scaled to 1e-35 (FP32 normal range, tiny) either
+    converges or falls back, and its answer matches the oracle."""
+    n, k = 48, 4
+    A = spd_batch(k, n, seed=33)
+    B = np.random.default_rng(4).standard_normal((k, 1, n))
+    B[0] *= 1e39                      # > FLT_MAX
+    B[1][0, 5] = np.nan
+    A[2] *= 1e-35
+    X, iters, info = api.posv_mixed_batched(dev(A), dev(B), tol=1e-12, max_iter=10)
+    it, inf = iters.cpu().tolist(), info.cpu().tolist()
+    X = X.cpu().numpy()
+    assert it[0] == -1 and it[1] == -1 and it[3] >= 0
+    assert inf == [0, 0, 0, 0]
+    for b in (0, 2, 3):
+        Lo, _ = oracle.potrf(A[b])
+        ref = oracle.potrs(Lo, B[b].T)
+        assert np.all(np.isfinite(X[b]))
+        assert np.max(np.abs(X[b].T - ref)) <= 1e-10 * np.abs(ref).max(), b
+    assert np.isnan(X[1]).any()
+
+
 def test_mixed_inputs_are_read_only():
     n, k = 64, 3
     A = spd_batch(k, n, seed=11)
@@ -242,6 +313,22 @@ def test_config_cholesky_mixed_full_size():
     Xp = tB2.cpu().numpy()
     assert backward_error(A, Xp, B) <= 1e-12
     assert np.max(np.abs(Xm - Xp)) <= 1e4 * 1e-12 * np.abs(Xp).max()
+
+
+def test_config_cholesky_mixed_sample_against_oracle(oracle):
+    """The configuration's own systems (lambda in [1, 1e4], cond 1e4), a
+    seeded sample of 96 of the 8192: the mixed solution equals the FP64
+    oracle's (dpotrf + dpotrs restated) to cond * 1e-12."""
+    from gputils_amd import workloads
+    A, B = workloads.cholesky_mixed()
+    X, iters, info = api.posv_mixed_batched(dev(A), dev(B), tol=1e-12)
+    Xm = X.cpu().numpy()
+    pick = np.sort(np.random.default_rng(8192).choice(A.shape[0], 96, replace=False))
+    As, Bs = A[pick].copy(), B[pick].copy()
+    assert oracle.posv_batched(As, Bs) == 0
+    err = np.abs(Xm[pick] - Bs).max(axis=(1, 2)) / np.abs(Bs).max(axis=(1, 2))
+    assert err.max() <= 1e4 * 1e-12, err.max()
+    assert backward_error(A[pick], Bs, B[pick]) <= 1e-13
 
 
 @pytest.mark.parametrize("n", [5, 32])

@@ -97,6 +97,25 @@ def test_gemm_many_batches_grid_split(oracle):
         assert np.allclose(C2[b].T, A2[b].T.astype(np.float64) @ A2[b].T, atol=1e-4)
 
 
+@pytest.mark.parametrize("mode,bound", [("fp32", 64 * 2.0 ** -24), ("bf16x3", 2.0 ** -14)])
+def test_config_gemm_full_size(mode, bound):
+    """The configuration itself: 16384 products of 64x64 by 64x64 FP32
+    matrices (U[-1,1], seeded), each mode checked on every entry against
+    the FP64 host product: max |dC| / (|A||B|) within the mode's bound
+    (FP32: k u32; split-BF16: 2^-14, twice the 2^-15 split error)."""
+    from gputils_amd import workloads
+    A, B = workloads.gemm()
+    C = api.gemm_batched(dev(A), dev(B), mode=mode).cpu().numpy().astype(np.float64)
+    A64, B64 = A.astype(np.float64), B.astype(np.float64)
+    # tensor layout (k, n, m): C^T = B^T A^T per matrix
+    ref = np.matmul(B64, A64)
+    mag = np.matmul(np.abs(B64), np.abs(A64))
+    err = np.max(np.abs(C - ref) / mag)
+    assert err <= bound, err
+    if mode == "bf16x3":
+        assert err > 64 * 2.0 ** -24 / 16  # the split error is visible: this is not the FP32 path
+
+
 @pytest.mark.parametrize("dtype,eps", [(torch.float32, G.PRECISION_LOW), (torch.float64, G.PRECISION_HIGH)])
 def test_reductions_golden(dtype, eps):
     a = dev(np.array(G.T234A, dtype=np.float64), dtype)

@@ -70,7 +70,8 @@ def test_rank_golden(dtype, eps):
 
 
 SHAPES = [(1, 1), (5, 1), (3, 2), (10, 4), (17, 9), (30, 30), (64, 64), (96, 48), (128, 64), (129, 40),
-          (200, 50), (300, 64), (1000, 20)]  # U in registers up to m = 128, in column chunks beyond
+          (200, 50), (300, 64), (1000, 20),  # U in registers up to m = 128, in column chunks beyond
+          (65, 65), (100, 96), (128, 128), (200, 130)]  # n > 64: G, W, V in the workspace
 
 
 @pytest.mark.parametrize("m,n", SHAPES)
@@ -98,7 +99,7 @@ def test_svd_matches_oracle(oracle, m, n, dtype):
         Pc, Po = U[b][:, n:] @ U[b][:, n:].T, Uo[:, n:] @ Uo[:, n:].T
         assert np.allclose(Pc, Po, atol=tol * 100)
         assert np.linalg.norm(U[b][:, :n] * S[b] @ VT[b] - A[b]) <= tol * 10 * smax * np.sqrt(m * n)
-    assert sw.max() <= 15
+    assert sw.max() <= 20
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
@@ -144,7 +145,7 @@ def test_svd_rank_deficient_and_zero(oracle, dtype):
     assert np.allclose(U[1], np.eye(20)) and np.allclose(VT[1], np.eye(8))
 
 
-@pytest.mark.parametrize("m,n", [(96, 48), (20, 8), (64, 64), (5, 1), (30, 17)])
+@pytest.mark.parametrize("m,n", [(96, 48), (20, 8), (64, 64), (5, 1), (30, 17), (120, 80), (96, 96)])
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 def test_svd_without_vt_same_values_and_subspaces(m, n, dtype):
     """VT = NULL (the nullspace builds' mode) runs the sweeps on R^T: the
@@ -184,8 +185,8 @@ def test_svd_argument_checks():
     api.gesvd_batched(A)
     with pytest.raises(api.GputilsError):
         api.gesvd_batched(torch.zeros((2, 5, 3), dtype=torch.float64, device="cuda"))  # fat
-    with pytest.raises(api.GputilsError):
-        api.gesvd_batched(torch.zeros((1, 65, 80), dtype=torch.float64, device="cuda"))  # n > 64
+    S, _, _, info = api.gesvd_batched(torch.zeros((1, 65, 80), dtype=torch.float64, device="cuda"))  # n > 64
+    assert info.tolist() == [0] and torch.equal(S, torch.zeros_like(S))
 
 
 # ------------------------------------------------------------ nullspace --
@@ -252,6 +253,68 @@ def test_nullspace_config(oracle):
     z = tb.cpu().double().numpy()[:, 0, :]
     res = np.einsum("kij,kj->ki", Am, z)
     assert np.max(np.linalg.norm(res, axis=1) / np.linalg.norm(x[:, 0, :], axis=1)) < 1e-5
+
+
+@pytest.mark.parametrize("m,n,rk", [(80, 120, 70), (96, 130, 96), (65, 65, 30)])
+def test_nullspace_of_wide_matrices(oracle, m, n, rk):
+    """More than 64 rows (the SVD of A^T then has more than 64 columns):
+    rank and P agree with the oracle; A N = 0 and N^T N = diag(I, 0)."""
+    k = 2
+    r = np.random.default_rng(m + n)
+    A = np.einsum("kir,krj->kij", r.standard_normal((k, m, rk)), r.standard_normal((k, rk, n)))
+    tA = dev(api.from_matrices(torch.from_numpy(A)).numpy())
+    N, P, rank = api.nullspace_batched(tA)
+    assert rank.cpu().tolist() == [rk] * k
+    Nm = api.to_matrices(N).cpu().numpy()
+    Pm = api.to_matrices(P).cpu().numpy()
+    nul = n - rk
+    for b in range(k):
+        _, Po, onul = oracle.nullspace(A[b])
+        assert onul == nul
+        assert np.max(np.abs(Pm[b] - Po)) < 1e-11
+        assert np.linalg.norm(A[b] @ Nm[b]) < 1e-10 * np.linalg.norm(A[b])
+        G_ = Nm[b].T @ Nm[b]
+        want = np.zeros((n, n))
+        want[:nul, :nul] = np.eye(nul)
+        assert np.max(np.abs(G_ - want)) < 1e-12
+
+
+def test_config_project_1000_calls_in_one_graph():
+    """The configuration's projection at full size: P of the 2048 rank-40
+    48 x 96 FP32 matrices; 1000 Nullspace::project calls captured in one HIP
+    graph and replayed twice, with eager work between the replays.  Each
+    replay equals 1000 eager calls bit for bit, and ||A P x|| <= 1e-5 ||x||
+    for every matrix (FP64 on the host)."""
+    from gputils_amd import workloads
+    A_t, x = workloads.svd_nullspace()
+    _, P, rank = api.nullspace_batched(dev(A_t))
+    assert rank.cpu().tolist() == [40] * 2048
+    x0 = dev(x)
+    eager = x0.clone()
+    for _ in range(1000):
+        api.project(P, eager)
+    b = x0.clone()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        api.project(P, b)            # warm-up outside capture
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g, stream=s):
+            for _ in range(1000):
+                api.project(P, b)
+    torch.cuda.synchronize()
+    Am = api.to_matrices(torch.from_numpy(A_t)).double().numpy()
+    for rep in range(2):
+        b.copy_(x0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(b, eager), rep
+        z = b.cpu().double().numpy()[:, 0, :]
+        res = np.einsum("kij,kj->ki", Am, z)
+        assert np.max(np.linalg.norm(res, axis=1) / np.linalg.norm(x[:, 0, :], axis=1)) <= 1e-5
+        _ = (x0 * 2).sum().item()   # eager work between replays
 
 
 def test_project_graph_replays_match_eager():
