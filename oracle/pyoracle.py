@@ -33,7 +33,8 @@ def build(force: bool = False) -> str:
     """Compile oracle.c with gcc (make) into liboracle.so next to this file,
     also when a source is newer than the library."""
     stale = not os.path.exists(_LIB_PATH) or any(
-        os.path.getmtime(os.path.join(_HERE, f)) > os.path.getmtime(_LIB_PATH) for f in ("oracle.c", "oracle.h"))
+        os.path.getmtime(os.path.join(_HERE, f)) > os.path.getmtime(_LIB_PATH)
+        for f in ("oracle.c", "oracle.h", "lapack_loop.c"))
     if force or stale:
         subprocess.run(["make", "-s", "-B", "-C", _HERE], check=True)
     return _LIB_PATH
