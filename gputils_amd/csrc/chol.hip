@@ -22,6 +22,7 @@
 //    Both MFMA operands of a 16x16x4 step read LDS at index lane + 64*kk.
 //  * Triangular solves use the same block inverses: per block row a 16x16
 //    matrix-vector product replaces the serial substitution.
+#include <mutex>
 #include <type_traits>
 
 #include "chol_mixed.hpp"
@@ -656,30 +657,20 @@ int64_t gputils_dposv_mixed_workspace_bytes(int64_t n, int64_t batch) {
     return n <= cholw::kMixedMaxN ? 0 : n * n * (int64_t)sizeof(double) * batch;
 }
 
-int gputils_dposv_mixed_batched(int64_t n, int64_t nrhs, const double *A, int64_t lda,
-                                int64_t strideA, const double *B, int64_t ldb, int64_t strideB,
-                                double *X, int64_t ldx, int64_t strideX, double tol, int max_iter,
-                                int *iters, int *info, void *work, int64_t batch, void *stream) {
-    GPUTILS_ARG(n >= 0, 1, "n < 0");
-    GPUTILS_ARG(nrhs >= 0, 2, "nrhs < 0");
-    GPUTILS_ARG(lda >= (n > 1 ? n : 1), 4, "lda < max(1,n)");
-    GPUTILS_ARG(ldb >= (n > 1 ? n : 1), 7, "ldb < max(1,n)");
-    GPUTILS_ARG(ldx >= (n > 1 ? n : 1), 10, "ldx < max(1,n)");
-    GPUTILS_ARG(tol > 0.0, 12, "tol <= 0");
-    GPUTILS_ARG(max_iter >= 0, 13, "max_iter < 0");
-    GPUTILS_ARG(batch >= 0 && batch <= 0x7fffffff, 17, "batch out of range");
-    if (n == 0 || nrhs == 0 || batch == 0) return GPUTILS_OK;
-    GPUTILS_ARG(A != nullptr, 3, "A is NULL");
-    GPUTILS_ARG(B != nullptr, 6, "B is NULL");
-    GPUTILS_ARG(X != nullptr, 9, "X is NULL");
-    GPUTILS_ARG(iters != nullptr, 14, "iters is NULL");
-    GPUTILS_ARG(info != nullptr, 15, "info is NULL");
+}  // extern "C"
+
+namespace {
+
+// Arguments of the mixed solve after their checks; work is the FP64 copy
+// of A for n > kMixedMaxN (the only device memory besides the caller's).
+int posv_mixed_run(int64_t n, int64_t nrhs, const double *A, int64_t lda, int64_t strideA, const double *B,
+                   int64_t ldb, int64_t strideB, double *X, int64_t ldx, int64_t strideX, double tol, int max_iter,
+                   int *iters, int *info, double *work, int64_t batch, void *stream) {
     hipStream_t st = as_stream(stream);
     if (n > cholw::kMixedMaxN) {
         // beyond the fused kernel: FP64 factor and solve of copies (A and B
         // stay read only), every system reported as solved by the fallback
-        GPUTILS_ARG(work != nullptr, 16, "work is NULL (query gputils_dposv_mixed_workspace_bytes)");
-        double *L = static_cast<double *>(work);
+        double *L = work;
         const unsigned blocks = (unsigned)(batch < 4096 ? batch : 4096);
         hipLaunchKernelGGL((chol::copy_batch_kernel<double>), dim3(blocks), dim3(chol::kThreads), 0, st, (int)n, (int)n, A,
                            lda, strideA, L, n, n * n, batch);
@@ -699,6 +690,96 @@ int gputils_dposv_mixed_batched(int64_t n, int64_t nrhs, const double *A, int64_
     // FP64 fallback for matrices flagged with iters < 0 (A, B read only)
     return chol::posv_dispatch<double, false>(n, nrhs, const_cast<double *>(A), lda, strideA, B, ldb,
                                               strideB, X, ldx, strideX, info, batch, iters, st);
+}
+
+// The argument checks both mixed entry points share (their argument
+// numbers agree up to info, 15).
+int posv_mixed_check(int64_t n, int64_t nrhs, const double *A, int64_t lda, const double *B, int64_t ldb,
+                     const double *X, int64_t ldx, double tol, int max_iter, const int *iters, const int *info,
+                     int64_t batch, int batch_arg) {
+    GPUTILS_ARG(n >= 0, 1, "n < 0");
+    GPUTILS_ARG(nrhs >= 0, 2, "nrhs < 0");
+    GPUTILS_ARG(lda >= (n > 1 ? n : 1), 4, "lda < max(1,n)");
+    GPUTILS_ARG(ldb >= (n > 1 ? n : 1), 7, "ldb < max(1,n)");
+    GPUTILS_ARG(ldx >= (n > 1 ? n : 1), 10, "ldx < max(1,n)");
+    GPUTILS_ARG(tol > 0.0, 12, "tol <= 0");
+    GPUTILS_ARG(max_iter >= 0, 13, "max_iter < 0");
+    GPUTILS_ARG(batch >= 0 && batch <= 0x7fffffff, batch_arg, "batch out of range");
+    if (n == 0 || nrhs == 0 || batch == 0) return GPUTILS_OK;
+    GPUTILS_ARG(A != nullptr, 3, "A is NULL");
+    GPUTILS_ARG(B != nullptr, 6, "B is NULL");
+    GPUTILS_ARG(X != nullptr, 9, "X is NULL");
+    GPUTILS_ARG(iters != nullptr, 14, "iters is NULL");
+    GPUTILS_ARG(info != nullptr, 15, "info is NULL");
+    return GPUTILS_OK;
+}
+
+// Scratch of gputils_dposv_mixed_batched_noalloc: the FP64 copies of A that
+// systems beyond the fused kernel (n > kMixedMaxN) are factorised in.  Each
+// GPU holds its own copy for the life of the process; every call on one GPU
+// uses the same array.
+constexpr int64_t kNoallocScratchBytes = 16ll << 20;
+static __device__ __align__(256) double scratch[kNoallocScratchBytes / sizeof(double)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_lock;
+double *g_scratch_addr[kMaxDevices] = {};  // device address of `scratch`, per device
+
+// The device address of `scratch` on the current device, looked up once per
+// device (the host name of the array is only a stand-in for it).
+int scratch_on_current_device(double **addr) {
+    int dev = -1;
+    GPUTILS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("device %d is beyond the %d devices the allocation-free entry point supports", dev, kMaxDevices);
+        return 1;
+    }
+    std::lock_guard<std::mutex> hold(g_scratch_lock);
+    if (g_scratch_addr[dev] == nullptr) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            set_error("hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+            return e != hipSuccess ? (int)e : 1;
+        }
+        g_scratch_addr[dev] = static_cast<double *>(p);
+    }
+    *addr = g_scratch_addr[dev];
+    return GPUTILS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gputils_dposv_mixed_batched(int64_t n, int64_t nrhs, const double *A, int64_t lda,
+                                int64_t strideA, const double *B, int64_t ldb, int64_t strideB,
+                                double *X, int64_t ldx, int64_t strideX, double tol, int max_iter,
+                                int *iters, int *info, void *work, int64_t batch, void *stream) {
+    int rc = posv_mixed_check(n, nrhs, A, lda, B, ldb, X, ldx, tol, max_iter, iters, info, batch, 17);
+    if (rc != GPUTILS_OK || n == 0 || nrhs == 0 || batch == 0) return rc;
+    GPUTILS_ARG(n <= cholw::kMixedMaxN || work != nullptr, 16,
+                "work is NULL (query gputils_dposv_mixed_workspace_bytes)");
+    return posv_mixed_run(n, nrhs, A, lda, strideA, B, ldb, strideB, X, ldx, strideX, tol, max_iter, iters, info,
+                          static_cast<double *>(work), batch, stream);
+}
+
+int gputils_dposv_mixed_batched_noalloc(int64_t n, int64_t nrhs, const double *A, int64_t lda, int64_t strideA,
+                                        const double *B, int64_t ldb, int64_t strideB, double *X, int64_t ldx,
+                                        int64_t strideX, double tol, int max_iter, int *iters, int *info,
+                                        int64_t batch, void *stream) {
+    int rc = posv_mixed_check(n, nrhs, A, lda, B, ldb, X, ldx, tol, max_iter, iters, info, batch, 16);
+    if (rc != GPUTILS_OK || n == 0 || nrhs == 0 || batch == 0) return rc;
+    double *work = nullptr;
+    if (n > cholw::kMixedMaxN) {
+        // n <= 1448 keeps n*n*8*batch inside int64 for any batch <= 2^31
+        GPUTILS_ARG(n <= 1448 && n * n * (int64_t)sizeof(double) * batch <= kNoallocScratchBytes, 16,
+                    "n*n*batch*8 bytes exceed the 16 MiB scratch of gputils_dposv_mixed_batched_noalloc");
+        rc = scratch_on_current_device(&work);
+        if (rc != GPUTILS_OK) return rc;
+    }
+    return posv_mixed_run(n, nrhs, A, lda, strideA, B, ldb, strideB, X, ldx, strideX, tol, max_iter, iters, info,
+                          work, batch, stream);
 }
 
 }  // extern "C"

@@ -180,6 +180,20 @@ int gputils_dposv_mixed_batched(int64_t n, int64_t nrhs, const double *A, int64_
                                 int64_t strideA, const double *B, int64_t ldb, int64_t strideB,
                                 double *X, int64_t ldx, int64_t strideX, double tol, int max_iter,
                                 int *iters, int *info, void *work, int64_t batch, void *stream);
+/* gputils_dposv_mixed_batched without the work argument, for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  No reference counterpart.  It
+ * allocates nothing: for n > 128 the FP64 copies of A go to a 16 MiB
+ * array of the library that each GPU holds for the life of the process,
+ * so n*n*batch*8 must not exceed 16 MiB (larger calls return -16 and
+ * launch nothing); n <= 128 needs no scratch.  Results are those of
+ * gputils_dposv_mixed_batched.  Calls on one GPU share the array: calls
+ * that overlap in time on one GPU (different streams, or threads) race.
+ * Every call overwrites what it uses before reading it. */
+int gputils_dposv_mixed_batched_noalloc(int64_t n, int64_t nrhs, const double *A, int64_t lda,
+                                        int64_t strideA, const double *B, int64_t ldb, int64_t strideB,
+                                        double *X, int64_t ldx, int64_t strideX, double tol, int max_iter,
+                                        int *iters, int *info, int64_t batch, void *stream);
 
 /* ------------------------------------------------------- QR / least sq --- */
 /* Batched least squares min ||A_i x - b_i|| for m >= n (cublas?gelsBatched
