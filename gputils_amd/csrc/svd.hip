@@ -27,11 +27,13 @@
 //      to 16 columns at a time with one 16-lane group per column.
 //
 // Where the matrices live (the kernel's MODE): kResident keeps G, W and V
-// in LDS; kGlobalG moves G to the global workspace when it does not fit;
-// kWide (n > 64, or m too large for either) keeps G, W and V in the global
-// workspace and LDS holds only the per-column scalars.  kResident and
-// kGlobalG hold each Jacobi pair's column entries in registers (n <= 64);
-// kWide loops over them.
+// in LDS; kParked factorises G in LDS, parks the reflectors in the global
+// workspace and lets W and V reuse G's LDS (half the LDS of kResident, so
+// twice the workgroups per CU); kGlobalG moves G to the global workspace
+// when it does not fit; kWide (n > 64, or m too large for the others) keeps
+// G, W and V in the global workspace and LDS holds only the per-column
+// scalars.  All but kWide hold each Jacobi pair's column entries in
+// registers (n <= 64); kWide loops over them.
 #include <algorithm>
 #include <cfloat>
 
@@ -51,6 +53,7 @@ constexpr int kMaxLds = 160 * 1024;
 constexpr int kResident = 0;  // G, W, V in LDS
 constexpr int kGlobalG = 1;   // G in the workspace, W, V in LDS
 constexpr int kWide = 2;      // G, W, V in the workspace
+constexpr int kParked = 3;    // G in LDS for the QR, then in the workspace; W, V over G's LDS
 
 // Storage layout, in doubles unless noted: offsets into LDS, or (kWide, and
 // G of kGlobalG) into the matrix's slice of the workspace.
@@ -73,14 +76,23 @@ __host__ __device__ inline Layout layout(int m, int n, int mode) {
     const int64_t mn = (int64_t)m * n, nn = (int64_t)n * n;
     L.ws = (nn > 4 * (int64_t)m) ? nn : 4 * (int64_t)m;
     int64_t o = 0, gofs = 0;  // LDS and workspace cursors
-    if (mode == kResident) {
+    if (mode == kParked) {
+        // one LDS region holds G during the QR, then W and V; the reflectors
+        // are parked in the workspace for the U phase
+        L.g = 0;
+        L.w = 0;
+        L.x = L.ws;
+        o = (mn > L.ws + nn) ? mn : L.ws + nn;
+        gofs = mn;
+    } else if (mode == kResident) {
         L.g = o;
         o += mn;
     } else {
         L.g = gofs;
         gofs += mn;
     }
-    if (mode == kWide) {
+    if (mode == kParked) {
+    } else if (mode == kWide) {
         L.w = gofs;
         gofs += L.ws;
         L.x = gofs;
@@ -240,7 +252,7 @@ __device__ void complete_basis_strided(double *X, int n, int nz, int lane) {
 }
 
 template <typename T, int MODE>
-__global__ __launch_bounds__(kThreads) void gesvd_kernel(int m, int n, const T *__restrict__ A, int64_t lda,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void gesvd_kernel(int m, int n, const T *__restrict__ A, int64_t lda,
                                                          int64_t strideA, T *S, int64_t strideS, T *U,
                                                          int64_t ldu, int64_t strideU, T *VT, int64_t ldvt,
                                                          int64_t strideVT, double *work, int *info,
@@ -251,7 +263,7 @@ __global__ __launch_bounds__(kThreads) void gesvd_kernel(int m, int n, const T *
     const Layout L = layout(m, n, MODE);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     double *ws = work + b * L.wstride;  // this matrix's workspace (unused by kResident)
-    double *G = MODE == kResident ? lds + L.g : ws + L.g;
+    double *G = (MODE == kResident || MODE == kParked) ? lds + L.g : ws + L.g;
     double *W = MODE == kWide ? ws + L.w : lds + L.w;
     double *X = MODE == kWide ? ws + L.x : lds + L.x;
     double *sig = lds + L.sig;
@@ -276,11 +288,39 @@ __global__ __launch_bounds__(kThreads) void gesvd_kernel(int m, int n, const T *
     // zero sigma, and on this matrix type (the nullspace builds' A^T, rank
     // deficient) they converge in fewer sweeps.
     const bool on_rt = (VT == nullptr);
-    for (int e = tid; e < n * n; e += kThreads) {
-        const int i = e % n, j = e / n;
-        if (on_rt) W[e] = (j <= i) ? G[j + (int64_t)i * m] : 0.0;
-        else W[e] = (i <= j) ? G[i + (int64_t)j * m] : 0.0;
-        X[e] = (i == j) ? 1.0 : 0.0;
+    if constexpr (MODE == kParked) {
+        // W and V take over G's LDS: R goes through registers and, when U
+        // is wanted, the reflectors to the workspace (read again for U only)
+        constexpr int kR = (kMaxN * kMaxN + kThreads - 1) / kThreads;
+        double rv[kR];
+#pragma unroll
+        for (int t = 0; t < kR; ++t) {
+            const int e = tid + kThreads * t, i = e % n, j = e / n;
+            rv[t] = 0.0;
+            if (e < n * n) {
+                if (on_rt) rv[t] = (j <= i) ? G[j + (int64_t)i * m] : 0.0;
+                else rv[t] = (i <= j) ? G[i + (int64_t)j * m] : 0.0;
+            }
+        }
+        if (U != nullptr)
+            for (int e = tid; e < m * n; e += kThreads) ws[e] = G[e];
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < kR; ++t) {
+            const int e = tid + kThreads * t, i = e % n, j = e / n;
+            if (e < n * n) {
+                W[e] = rv[t];
+                X[e] = (i == j) ? 1.0 : 0.0;
+            }
+        }
+        G = ws;
+    } else {
+        for (int e = tid; e < n * n; e += kThreads) {
+            const int i = e % n, j = e / n;
+            if (on_rt) W[e] = (j <= i) ? G[j + (int64_t)i * m] : 0.0;
+            else W[e] = (i <= j) ? G[i + (int64_t)j * m] : 0.0;
+            X[e] = (i == j) ? 1.0 : 0.0;
+        }
     }
     __syncthreads();
 
@@ -472,15 +512,17 @@ __global__ __launch_bounds__(kThreads) void gesvd_kernel(int m, int n, const T *
     const int g16 = tid >> 4, l16 = tid & 15;
     T *u = U + b * strideU;
     if (m <= 128) {
-        // every column at once in registers: 16-lane group g16 holds
-        // columns g16 + 16 q, lane l16 rows l16 + 16 k; each reflector is
-        // read once (rows clamped into the column, masked to [0..0, 1, v])
-        // and its eight column updates reduce together
-        constexpr int kQ = 8, kK = 8;
+        // 64 columns at a time in registers (one pass for m <= 64, two up
+        // to 128): 16-lane group g16 holds columns c0 + g16 + 16 q, lane l16
+        // rows l16 + 16 k; each reflector is read once per pass (rows
+        // clamped into the column, masked to [0..0, 1, v]) and its four
+        // column updates reduce together
+        constexpr int kQ = 4, kK = 8;
+        for (int c0 = 0; c0 < m; c0 += 16 * kQ) {
         double xr[kQ][kK];
 #pragma unroll
         for (int q = 0; q < kQ; ++q) {
-            const int cc = g16 + 16 * q;
+            const int cc = c0 + g16 + 16 * q;
 #pragma unroll
             for (int k = 0; k < kK; ++k) {
                 const int i = l16 + 16 * k;
@@ -518,12 +560,13 @@ __global__ __launch_bounds__(kThreads) void gesvd_kernel(int m, int n, const T *
         }
 #pragma unroll
         for (int q = 0; q < kQ; ++q) {
-            const int cc = g16 + 16 * q;
+            const int cc = c0 + g16 + 16 * q;
 #pragma unroll
             for (int k = 0; k < kK; ++k) {
                 const int i = l16 + 16 * k;
                 if (cc < m && i < m) u[i + (int64_t)cc * ldu] = (T)xr[q][k];
             }
+        }
         }
         return;
     }
@@ -646,8 +689,14 @@ __global__ __launch_bounds__(kThreads) void nullspace_build_kernel(int n, const 
 // Where a matrix of m x n lives: in LDS when it fits, G in the workspace
 // next, everything in the workspace for n > kMaxN or very tall matrices.
 inline int gesvd_mode(int m, int n) {
-    if (n <= kMaxN && layout(m, n, kResident).bytes <= kMaxLds) return kResident;
-    if (n <= kMaxN && layout(m, n, kGlobalG).bytes <= kMaxLds) return kGlobalG;
+    if (n <= kMaxN) {
+        // the most workgroups per CU (LDS-bound; resident on ties)
+        const int res = layout(m, n, kResident).bytes, park = layout(m, n, kParked).bytes;
+        const int per_res = res <= kMaxLds ? kMaxLds / res : 0, per_park = park <= kMaxLds ? kMaxLds / park : 0;
+        if (per_res > 0 && per_res >= per_park) return kResident;
+        if (per_park > 0) return kParked;
+        if (layout(m, n, kGlobalG).bytes <= kMaxLds) return kGlobalG;
+    }
     return kWide;
 }
 
@@ -690,6 +739,9 @@ int gesvd_batched(int m, int n, const T *A, int64_t lda, int64_t strideA, T *S, 
                                           nullptr, info, sweeps, batch, st);
     GPUTILS_ARG(layout(m, n, mode).bytes <= kMaxLds, 1, "m too large for the batched SVD");
     GPUTILS_ARG(work != nullptr, 14, "work is NULL (query gputils_gesvd_workspace_bytes)");
+    if (mode == kParked)
+        return launch_gesvd<T, kParked>(m, n, A, lda, strideA, S, strideS, U, ldu, strideU, VT, ldvt, strideVT,
+                                        static_cast<double *>(work), info, sweeps, batch, st);
     if (mode == kGlobalG)
         return launch_gesvd<T, kGlobalG>(m, n, A, lda, strideA, S, strideS, U, ldu, strideU, VT, ldvt, strideVT,
                                          static_cast<double *>(work), info, sweeps, batch, st);
