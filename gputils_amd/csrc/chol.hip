@@ -10,7 +10,7 @@
 // Design (one 256-thread workgroup = 4 wavefronts per matrix):
 //  * The lower block triangle of the matrix lives in LDS as 16x16 blocks
 //    (block (I,J), I >= J, at index I(I+1)/2 + J, column-major inside the
-//    block).  n is padded to a multiple of 16 with an identity tail, so the
+//    block with the row index XOR-ed by the column, see sw()).  n is padded to a multiple of 16 with an identity tail, so the
 //    padded factor is diag(L, I) and no kernel path needs edge masks.
 //  * Right-looking blocked algorithm, per block column J:
 //      1. wave 0 factors the 16x16 diagonal block in registers (one row per
@@ -19,7 +19,8 @@
 //         when the factor is an output);
 //      2. panel   L_IJ = A_IJ D_J^T      (16x16x4 MFMA, 4 per block);
 //      3. update  A_IK -= L_IJ L_KJ^T    (16x16x4 MFMA, 4 per tile).
-//    Both MFMA operands of a 16x16x4 step read LDS at index lane + 64*kk.
+//    Both MFMA operands of a 16x16x4 step read element
+//    (lane % 16, lane / 16 + 4 kk) of their block.
 //  * Triangular solves use the same block inverses: per block row a 16x16
 //    matrix-vector product replaces the serial substitution.
 #include <mutex>
@@ -37,6 +38,14 @@ constexpr int kWaves = kThreads / kWave;
 constexpr int kRhsChunk = 4;  // right-hand sides per solve sweep (lane = row + 16*rhs)
 
 __host__ __device__ constexpr int tri(int I) { return (I * (I + 1)) >> 1; }
+
+// Element (r, c) of a 16x16 block in LDS sits at sw(r, c): column-major with
+// the row index XOR-ed by the column.  A column (fixed c) still fills 16
+// consecutive slots, and a row (fixed r, c = 0..15), which the MFMA
+// accumulator layout and the transposed solves read and write, falls in
+// 16 different banks instead of one (a plain column-major 16x16 block of
+// doubles puts a row on a 128-byte stride).
+__device__ __forceinline__ int sw(int r, int c) { return (r ^ c) + 16 * c; }
 
 // Load the lower triangle of the n x n matrix A (column-major, lda) into
 // LDS block storage with identity padding up to 16*nb.  Thread t owns
@@ -60,7 +69,7 @@ __device__ __forceinline__ void load_lower(const Tin *__restrict__ A, int64_t ld
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u)
-            if (b0 + u < nblk) L[(b0 + u) * 256 + i + 16 * k] = v[u];
+            if (b0 + u < nblk) L[(b0 + u) * 256 + sw(i, k)] = v[u];
         I = Iu;
         J = Ju;
     }
@@ -74,7 +83,7 @@ __device__ __forceinline__ void store_offdiag(T *__restrict__ A, int64_t lda, in
         const int gi = 16 * I + i;
         for (int J = 0; J < I; ++J) {
             const int gk = 16 * J + k;
-            if (gi < n && gk < n) A[gi + (int64_t)gk * lda] = L[(tri(I) + J) * 256 + i + 16 * k];
+            if (gi < n && gk < n) A[gi + (int64_t)gk * lda] = L[(tri(I) + J) * 256 + sw(i, k)];
         }
     }
 }
@@ -121,7 +130,7 @@ __device__ __forceinline__ int factor_diag_block(T *blk, T *Lout, int64_t ldo, i
     asm volatile("" : "+v"(lane));
     T row[16], invd[16];
 #pragma unroll
-    for (int c = 0; c < 16; ++c) row[c] = blk[(lane & 15) + 16 * c];
+    for (int c = 0; c < 16; ++c) row[c] = blk[sw(lane & 15, c)];
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
         const T d = readlane(row[k], k);
@@ -146,7 +155,7 @@ __device__ __forceinline__ int factor_diag_block(T *blk, T *Lout, int64_t ldo, i
     tri_inverse_cols(row, invd, x);
     if (lane < 16) {
 #pragma unroll
-        for (int r = 0; r < 16; ++r) blk[r + 16 * lane] = x[r];
+        for (int r = 0; r < 16; ++r) blk[sw(r, lane)] = x[r];
     }
     return -1;
 }
@@ -158,24 +167,25 @@ __device__ __forceinline__ void tile_update2(T *c0, const T *a0, const T *b0, T 
                                     const T *b1, bool two) {
     using M = Mfma16<T>;
     const int lane = threadIdx.x & 63;
-    const int col = 16 * (lane & 15);
+    const int col = lane & 15;
     typename M::acc_t x0, x1;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) x0[r] = c0[M::drow(lane, r) + col];
+    for (int r = 0; r < 4; ++r) x0[r] = c0[sw(M::drow(lane, r), col)];
     if (two) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) x1[r] = c1[M::drow(lane, r) + col];
+        for (int r = 0; r < 4; ++r) x1[r] = c1[sw(M::drow(lane, r), col)];
     }
 #pragma unroll
     for (int kk = 0; kk < 4; ++kk) {
-        x0 = M::fma(-a0[lane + 64 * kk], b0[lane + 64 * kk], x0);
-        if (two) x1 = M::fma(-a1[lane + 64 * kk], b1[lane + 64 * kk], x1);
+        const int o = sw(col, (lane >> 4) + 4 * kk);  // operand element (lane % 16, lane / 16 + 4 kk)
+        x0 = M::fma(-a0[o], b0[o], x0);
+        if (two) x1 = M::fma(-a1[o], b1[o], x1);
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) c0[M::drow(lane, r) + col] = x0[r];
+    for (int r = 0; r < 4; ++r) c0[sw(M::drow(lane, r), col)] = x0[r];
     if (two) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) c1[M::drow(lane, r) + col] = x1[r];
+        for (int r = 0; r < 4; ++r) c1[sw(M::drow(lane, r), col)] = x1[r];
     }
 }
 
@@ -204,11 +214,14 @@ __device__ __forceinline__ int factor_blocked(T *L, int nb, int n, T *Lout, int6
         for (int I = J + 1 + wave; I < nb; I += kWaves) {
             T *t = blk(I, J);
             typename M::acc_t acc = M::zero();
+            const int col = lane & 15;
 #pragma unroll
-            for (int kk = 0; kk < 4; ++kk) acc = M::fma(t[lane + 64 * kk], dJ[lane + 64 * kk], acc);
-            const int col = 16 * (lane & 15);
+            for (int kk = 0; kk < 4; ++kk) {
+                const int o = sw(col, (lane >> 4) + 4 * kk);
+                acc = M::fma(t[o], dJ[o], acc);
+            }
 #pragma unroll
-            for (int r = 0; r < 4; ++r) t[M::drow(lane, r) + col] = acc[r];
+            for (int r = 0; r < 4; ++r) t[sw(M::drow(lane, r), col)] = acc[r];
         }
         __syncthreads();
         if (J + 1 == nb) break;
@@ -254,25 +267,26 @@ __device__ __forceinline__ void invert_diag_blocks(T *L, int nb) {
         T *dJ = L + (tri(J) + J) * 256;
         T row[16], invd[16], x[16];
 #pragma unroll
-        for (int c = 0; c < 16; ++c) row[c] = dJ[(lane & 15) + 16 * c];
+        for (int c = 0; c < 16; ++c) row[c] = dJ[sw(lane & 15, c)];
 #pragma unroll
         for (int r = 0; r < 16; ++r) invd[r] = T(1) / readlane(row[r], r);
         tri_inverse_cols(row, invd, x);
         if (lane < 16) {
 #pragma unroll
-            for (int r = 0; r < 16; ++r) dJ[r + 16 * lane] = x[r];
+            for (int r = 0; r < 16; ++r) dJ[sw(r, lane)] = x[r];
         }
     }
 }
 
-// y_i = sum_k M[i + 16k] * v[k] (M = D or L block), two partial sums.
-template <typename T>
-__device__ __forceinline__ T dot16(const T *m, int i, int stride_i, int stride_k, const T *v) {
+// y_i = sum_k M(i, k) v[k], or sum_k M(k, i) v[k] when TRANS (M = a D or L
+// block), two partial sums.
+template <bool TRANS, typename T>
+__device__ __forceinline__ T dot16(const T *m, int i, const T *v) {
     T a0 = T(0), a1 = T(0);
 #pragma unroll
     for (int k = 0; k < 16; k += 2) {
-        a0 += m[i * stride_i + k * stride_k] * v[k];
-        a1 += m[i * stride_i + (k + 1) * stride_k] * v[k + 1];
+        a0 += m[TRANS ? sw(k, i) : sw(i, k)] * v[k];
+        a1 += m[TRANS ? sw(k + 1, i) : sw(i, k + 1)] * v[k + 1];
     }
     return a0 + a1;
 }
@@ -291,7 +305,7 @@ __device__ __forceinline__ void solve_blocked(const T *L, int nb, T *Y) {
     // forward: L y = b
     if (wave == 0) {
         T *y = Y + ld * r;
-        const T v = dot16(blk(0, 0), i, 1, 16, y);
+        const T v = dot16<false>(blk(0, 0), i, y);
         y[i] = v;
     }
     __syncthreads();
@@ -300,15 +314,15 @@ __device__ __forceinline__ void solve_blocked(const T *L, int nb, T *Y) {
         const int owner = (I + 1) & (kWaves - 1);
         if (wave == owner) {
             T *x = Y + 16 * (I + 1) + ld * r;
-            x[i] -= dot16(blk(I + 1, I), i, 1, 16, yI);
-            const T v = dot16(blk(I + 1, I + 1), i, 1, 16, x);
+            x[i] -= dot16<false>(blk(I + 1, I), i, yI);
+            const T v = dot16<false>(blk(I + 1, I + 1), i, x);
             x[i] = v;
         } else {
             int t = 0;
             for (int K = I + 2; K < nb; ++K, ++t) {
                 if ((t % (kWaves - 1)) != ((wave - owner - 1) & (kWaves - 1))) continue;
                 T *x = Y + 16 * K + ld * r;
-                x[i] -= dot16(blk(K, I), i, 1, 16, yI);
+                x[i] -= dot16<false>(blk(K, I), i, yI);
             }
         }
         __syncthreads();
@@ -317,7 +331,7 @@ __device__ __forceinline__ void solve_blocked(const T *L, int nb, T *Y) {
     const int last = nb - 1;
     if (wave == (last & (kWaves - 1))) {
         T *y = Y + 16 * last + ld * r;
-        const T v = dot16(blk(last, last), i, 16, 1, y);
+        const T v = dot16<true>(blk(last, last), i, y);
         y[i] = v;
     }
     __syncthreads();
@@ -326,15 +340,15 @@ __device__ __forceinline__ void solve_blocked(const T *L, int nb, T *Y) {
         const int owner = (I - 1) & (kWaves - 1);
         if (wave == owner) {
             T *x = Y + 16 * (I - 1) + ld * r;
-            x[i] -= dot16(blk(I, I - 1), i, 16, 1, zI);
-            const T v = dot16(blk(I - 1, I - 1), i, 16, 1, x);
+            x[i] -= dot16<true>(blk(I, I - 1), i, zI);
+            const T v = dot16<true>(blk(I - 1, I - 1), i, x);
             x[i] = v;
         } else {
             int t = 0;
             for (int K = I - 2; K >= 0; --K, ++t) {
                 if ((t % (kWaves - 1)) != ((wave - owner - 1) & (kWaves - 1))) continue;
                 T *x = Y + 16 * K + ld * r;
-                x[i] -= dot16(blk(I, K), i, 16, 1, zI);
+                x[i] -= dot16<true>(blk(I, K), i, zI);
             }
         }
         __syncthreads();
