@@ -323,6 +323,9 @@ def _traffic(kernel):
     except (OSError, ValueError):
         return None
     kernels = tab.get("kernels", {})
+    if "+" in kernel:  # one launch of each listed kernel per step
+        vals = [_traffic(k) for k in kernel.split("+")]
+        return sum(vals) if None not in vals else None
     if kernel.endswith("*"):
         vals = [v.get("hbm_bytes_per_launch") for k, v in kernels.items() if k.startswith(kernel[:-1])]
         return sum(vals) if vals and None not in vals else None
