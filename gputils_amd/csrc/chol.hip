@@ -489,13 +489,17 @@ __global__ __launch_bounds__(kThreads) void potrf_global_kernel(int n, T *A, int
     if (threadIdx.x == 0 && info != nullptr) info[b] = s_fail;
 }
 
+// skip != nullptr: systems with skip[b] != 0 (a failed factorisation) are
+// left as they are, as LAPACK dposv leaves B.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void potrs_global_kernel(int n, int nrhs, const T *L,
                                                                 int64_t lda, int64_t strideL, T *B,
                                                                 int64_t ldb, int64_t strideB,
-                                                                int *info, int64_t batch) {
+                                                                int *info, int64_t batch,
+                                                                const int *skip = nullptr) {
     const int64_t b = blockIdx.x;
     if (b >= batch) return;
+    if (skip != nullptr && skip[b] != 0) return;
     const T *l = L + b * strideL;
     for (int r = 0; r < nrhs; ++r) {
         T *x = B + b * strideB + (int64_t)r * ldb;
@@ -546,9 +550,9 @@ int potrf_dispatch(int64_t n, T *A, int64_t lda, int64_t strideA, int *info, int
     if (n <= 32)
         return chols::launch_posv_small<T>((int)n, 0, A, lda, strideA, A, lda, strideA, A, lda, strideA, info, batch,
                                            nullptr, true, st);
+    // n > 32 from here on (smaller n returned above)
     const dim3 grid((unsigned)batch), block(kThreads);
-    if (n <= 16) hipLaunchKernelGGL((potrf_kernel<T, 1>), grid, block, 0, st, (int)n, A, lda, strideA, info, batch);
-    else if (n <= 32) hipLaunchKernelGGL((potrf_kernel<T, 2>), grid, block, 0, st, (int)n, A, lda, strideA, info, batch);
+    if (n <= 48) hipLaunchKernelGGL((potrf_kernel<T, 3>), grid, block, 0, st, (int)n, A, lda, strideA, info, batch);
     else if (n <= 64) hipLaunchKernelGGL((potrf_kernel<T, 4>), grid, block, 0, st, (int)n, A, lda, strideA, info, batch);
     else if (n <= 128) hipLaunchKernelGGL((potrf_kernel<T, 8>), grid, block, 0, st, (int)n, A, lda, strideA, info, batch);
     else hipLaunchKernelGGL((potrf_global_kernel<T>), grid, block, 0, st, (int)n, A, lda, strideA, info, batch);
@@ -563,8 +567,8 @@ int potrs_dispatch(int64_t n, int64_t nrhs, const T *L, int64_t lda, int64_t str
     const dim3 grid((unsigned)batch), block(kThreads);
 #define GPUTILS_POTRS(NBV) \
     hipLaunchKernelGGL((potrs_kernel<T, NBV>), grid, block, 0, st, (int)n, (int)nrhs, L, lda, strideL, B, ldb, strideB, info, batch)
-    if (n <= 16) GPUTILS_POTRS(1);
-    else if (n <= 32) GPUTILS_POTRS(2);
+    // n > 32 from here on (smaller n returned above)
+    if (n <= 48) GPUTILS_POTRS(3);
     else if (n <= 64) GPUTILS_POTRS(4);
     else if (n <= 128) GPUTILS_POTRS(8);
     else hipLaunchKernelGGL((potrs_global_kernel<T>), grid, block, 0, st, (int)n, (int)nrhs, L, lda, strideL, B, ldb, strideB, info, batch);
@@ -586,8 +590,9 @@ int posv_dispatch(int64_t n, int64_t nrhs, T *A, int64_t lda, int64_t strideA, c
 #define GPUTILS_POSV(NBV)                                                                     \
     hipLaunchKernelGGL((posv_kernel<T, NBV, WRITE_L>), grid, block, 0, st, (int)n, (int)nrhs, A, \
                        lda, strideA, B, ldb, strideB, X, ldx, strideX, info, batch, mask)
-    if (n <= 16) GPUTILS_POSV(1);
-    else if (n <= 32) GPUTILS_POSV(2);
+    // n > 32 from here on (smaller n returned above); n <= 128 (checked by
+    // the callers)
+    if (n <= 48) GPUTILS_POSV(3);
     else if (n <= 64) GPUTILS_POSV(4);
     else GPUTILS_POSV(8);
 #undef GPUTILS_POSV
@@ -654,7 +659,7 @@ int gputils_dpotrs_batched(int64_t n, int64_t nrhs, const double *L, int64_t lda
 int gputils_dposv_batched(int64_t n, int64_t nrhs, double *A, int64_t lda, int64_t strideA,
                           double *B, int64_t ldb, int64_t strideB, int *info, int64_t batch,
                           void *stream) {
-    GPUTILS_ARG(n >= 0 && n <= 128, 1, "n must be in [0, 128]");
+    GPUTILS_ARG(n >= 0, 1, "n < 0");
     GPUTILS_ARG(nrhs >= 0, 2, "nrhs < 0");
     GPUTILS_ARG(lda >= (n > 1 ? n : 1), 4, "lda < max(1,n)");
     GPUTILS_ARG(ldb >= (n > 1 ? n : 1), 7, "ldb < max(1,n)");
@@ -662,8 +667,19 @@ int gputils_dposv_batched(int64_t n, int64_t nrhs, double *A, int64_t lda, int64
     if (n == 0 || batch == 0) return GPUTILS_OK;
     GPUTILS_ARG(A != nullptr, 3, "A is NULL");
     GPUTILS_ARG(nrhs == 0 || B != nullptr, 6, "B is NULL");
+    hipStream_t st = as_stream(stream);
+    if (n > 128) {
+        // global-memory factor, then the solve of the systems it factorised
+        // (a failed one keeps its B); info holds the factor's status
+        GPUTILS_ARG(info != nullptr, 9, "info is NULL (needed for n > 128)");
+        const int rc = chol::potrf_dispatch<double>(n, A, lda, strideA, info, batch, st);
+        if (rc != GPUTILS_OK || nrhs == 0) return rc;
+        hipLaunchKernelGGL((chol::potrs_global_kernel<double>), dim3((unsigned)batch), dim3(chol::kThreads), 0, st,
+                           (int)n, (int)nrhs, A, lda, strideA, B, ldb, strideB, (int *)nullptr, batch, info);
+        return launch_check("potrs_global_kernel");
+    }
     return chol::posv_dispatch<double, true>(n, nrhs, A, lda, strideA, B, ldb, strideB, B, ldb,
-                                             strideB, info, batch, nullptr, as_stream(stream));
+                                             strideB, info, batch, nullptr, st);
 }
 
 int64_t gputils_dposv_mixed_workspace_bytes(int64_t n, int64_t batch) {

@@ -444,7 +444,8 @@ __device__ __forceinline__ void trsv_upper_trans_regs(const float *S, int m, int
 // FP64, then `steps` refinements of the augmented system
 // [I A; A^T 0][r; x] = [b; 0] (Bjorck 1967): f = b - r - A x, g = -A^T r
 // in FP64, and with the FP32 factors h = R^{-T} g, d = Q^T f,
-// dx = R^{-1}(d[0:n] - h), dr = Q [h; d[n:m]].  This recovers FP64
+// dx = R^{-1}(d[0:n] - h), dr = Q [h; d[n:m]].  In the first step f is
+// b - r0 - A x0 = 0 up to rounding and is not formed.  This recovers FP64
 // accuracy also for inconsistent systems, where refining x alone stalls at
 // the FP32 level.  x overwrites b[0:n].  Global loads are batched (eight
 // per thread in flight) because one wavefront per SIMD leaves little else
@@ -524,18 +525,24 @@ __global__ __launch_bounds__(NW * kWave) void gels_ragged_kernel(const RaggedIte
         rows_minus_ax<NT>(a, m, n, xd, bd, nullptr, rd);
         for (int s = 0; s < steps; ++s) {
             __syncthreads();
-            // f = b - r - A x  -> c (FP32), staged through h-sized chunks of rows
+            // f = b - r - A x  -> c (FP32).  In the first step r = r0 was just
+            // formed as b - A x0, so f is zero up to FP64 rounding: it is
+            // taken as zero (no pass over A, and Q^T f = 0 below).
+            const bool f_zero = (s == 0);
             for (int i = threadIdx.x; i < m; i += NT) {
-                double f = bd[i] - rd[i];
-                int k = 0;
-                for (; k + 8 <= n; k += 8) {
-                    double av[8];
+                double f = 0.0;
+                if (!f_zero) {
+                    f = bd[i] - rd[i];
+                    int k = 0;
+                    for (; k + 8 <= n; k += 8) {
+                        double av[8];
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) av[u] = a[i + (int64_t)(k + u) * m];
+                        for (int u = 0; u < 8; ++u) av[u] = a[i + (int64_t)(k + u) * m];
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) f = fma(-av[u], xd[k + u], f);
+                        for (int u = 0; u < 8; ++u) f = fma(-av[u], xd[k + u], f);
+                    }
+                    for (; k < n; ++k) f = fma(-a[i + (int64_t)k * m], xd[k], f);
                 }
-                for (; k < n; ++k) f = fma(-a[i + (int64_t)k * m], xd[k], f);
                 c[i] = (float)f;
             }
             // g = -A^T r -> h (FP32); one wavefront per column, eight rows
@@ -558,8 +565,8 @@ __global__ __launch_bounds__(NW * kWave) void gels_ragged_kernel(const RaggedIte
             __syncthreads();
             // the two independent serial pieces of each half run on two
             // wavefronts at once
-            if (wave == 0) trsv_upper_trans_regs(S, m, n, h);       // h = R^{-T} g
-            if (wave == 1) apply_q_regs(S, m, n, tau, c, true);   // c = Q^T f
+            if (wave == 0) trsv_upper_trans_regs(S, m, n, h);                 // h = R^{-T} g
+            if (wave == 1 && !f_zero) apply_q_regs(S, m, n, tau, c, true);   // c = Q^T f
             __syncthreads();
             for (int i = threadIdx.x; i < n; i += NT) {
                 dx[i] = c[i] - h[i];

@@ -157,7 +157,10 @@ int gputils_dpotrs_batched(int64_t n, int64_t nrhs, const double *L, int64_t lda
                            double *B, int64_t ldb, int64_t strideB, int *info, int64_t batch,
                            void *stream);
 /* Fused factor + solve in FP64: A_i <- L_i, B_i <- A_i^{-1} B_i in one
- * launch (the pair factorise(); solve(b) of CholeskyBatchFactoriser). */
+ * launch (the pair factorise(); solve(b) of CholeskyBatchFactoriser).
+ * info[i] = 0 or the order of the first non-positive leading minor; a
+ * system whose factorisation fails keeps its B_i (as LAPACK dposv) for
+ * n > 128, where info must not be NULL. */
 int gputils_dposv_batched(int64_t n, int64_t nrhs, double *A, int64_t lda, int64_t strideA,
                           double *B, int64_t ldb, int64_t strideB, int *info, int64_t batch,
                           void *stream);

@@ -156,8 +156,8 @@ def test_potrs_matches_oracle(oracle, n, nrhs, dtype):
         assert np.max(np.abs(got[b].T - ref)) <= tol * max(1.0, np.abs(ref).max()), (n, nrhs, b)
 
 
-@pytest.mark.parametrize("n,nrhs", [(1, 1), (3, 1), (16, 2), (17, 3), (31, 1), (32, 1), (32, 5), (50, 4), (128, 4),
-                                    (128, 6)])
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (3, 1), (16, 2), (17, 3), (31, 1), (32, 1), (32, 5), (33, 2), (48, 3),
+                                    (50, 4), (128, 4), (128, 6), (129, 2), (180, 3)])
 def test_fused_posv_matches_oracle(oracle, n, nrhs):
     k = 7   # odd: the n <= 32 kernel pairs matrices per wavefront
     A = spd_batch(k, n, seed=200 + n)
@@ -172,6 +172,27 @@ def test_fused_posv_matches_oracle(oracle, n, nrhs):
         assert np.max(np.abs(np.tril(L[b]) - np.tril(Lo))) <= 1e-12 * np.abs(Lo).max()
         ref = oracle.potrs(Lo, B[b].T)
         assert np.max(np.abs(X[b].T - ref)) <= 1e-11 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("n", [40, 140])
+def test_fused_posv_failure_reports_info(oracle, n):
+    """A non-SPD system reports the oracle's info (blocked kernel for
+    n <= 128, global-memory path beyond); beyond 128 its B is left as it
+    was (LAPACK dposv), and its neighbours are solved."""
+    k = 3
+    A = spd_batch(k, n, seed=600 + n)
+    A[1][9, 9] = -2.0
+    B = np.random.default_rng(n).standard_normal((k, 2, n))
+    tA, tB = dev(A), dev(B)
+    info = api.posv_batched(tA, tB).cpu().tolist()
+    assert info == [oracle.potrf(A[b])[1] for b in range(k)] == [0, 10, 0]
+    X = tB.cpu().numpy()
+    for b in (0, 2):
+        Lo, _ = oracle.potrf(A[b])
+        ref = oracle.potrs(Lo, B[b].T)
+        assert np.max(np.abs(X[b].T - ref)) <= 1e-11 * np.abs(ref).max()
+    if n > 128:
+        assert np.array_equal(X[1], B[1])
 
 
 # ---------------------------------------------------- mixed precision --
