@@ -579,22 +579,47 @@ def bench_svd_nullspace(a):
     _, _, _, _, sw_full = api.gesvd_batched(At, compute_u=True, sweeps=True)
     tsvd_full = statistics.fmean(_events_time_ms(lambda: api.gesvd_batched(At, compute_u=True), MIN_EVENT_REPS))
     torch.cuda.synchronize()
-    # FP64 flops the SVD kernel executes per matrix (A^T is mm x nn, mm = 96,
-    # nn = 48): Householder QR 2 mm nn^2 - 2 nn^3 / 3; each Jacobi sweep on the
-    # nn x nn triangle, every pair counted as rotated: three dot products and
-    # the rotation of two W and two V columns, 18 nn flops per pair, nn (nn-1)/2
-    # pairs; U = Q diag(U_R, I): 4 (mm - j) mm flops per reflector j.
+    # the same SVD with every sweep in FP64, for comparison
+    prev = api.set_fp32_sweeps(False)
+    try:
+        _, _, _, _, sw_64 = api.gesvd_batched(At, compute_u=True, sweeps=True, compute_v=False)
+        tsvd_64 = statistics.fmean(_events_time_ms(lambda: api.gesvd_batched(At, compute_u=True, compute_v=False),
+                                                   MIN_EVENT_REPS))
+    finally:
+        api.set_fp32_sweeps(prev)
+    torch.cuda.synchronize()
+    # flops the SVD kernel executes per matrix (A^T is mm x nn, mm = 96,
+    # nn = 48): Householder QR 2 mm nn^2 - 2 nn^3 / 3 (FP64); each Jacobi
+    # sweep on the nn x nn triangle, every pair counted as rotated: three dot
+    # products and the rotation of two W and two V columns, 18 nn flops per
+    # pair, nn (nn-1)/2 pairs (FP32 or FP64 as the sweep runs); the step
+    # between them, F = X X^T, X - F X / 2 and W0 X (lower triangular W0),
+    # 5 nn^3 flops in FP64; U = Q diag(U_R, I): 4 (mm - j) mm flops per
+    # reflector j (FP64).
     mm, nn = n, m
-    sweeps = sw.cpu().numpy().astype(np.float64)
+    s64 = (sw & 0xFFFF).cpu().numpy().astype(np.float64)
+    s32 = (sw >> 16).cpu().numpy().astype(np.float64)
     qr_f = 2 * mm * nn ** 2 - 2 * nn ** 3 / 3
     sweep_f = 18 * nn * nn * (nn - 1) / 2
+    step_f = 5 * nn ** 3
     u_f = sum(4 * (mm - j) * mm for j in range(nn))
-    svd_flops = float(np.sum(qr_f + sweeps * sweep_f + u_f))
-    svd_roof = _roofline(("fp64 vector", FP64_PEAK_TFS), svd_flops, tsvd, "gesvd_kernel<float,0>",
-                         "FP64 flops executed: Householder QR of the 96x48 A^T, the measured Jacobi sweeps on the "
-                         "48x48 R^T (18*48 flops per pair, all 1128 pairs counted), U = Q diag(U_R, I)")
-    svd_roof["sweeps_max"] = int(sweeps.max())
-    svd_roof["sweeps_mean"] = float(sweeps.mean())
+    f64 = float(np.sum(qr_f + s64 * sweep_f + (s32 > 0) * step_f + u_f))
+    f32 = float(np.sum(s32 * sweep_f))
+    # the time these flops need at the FP64 and FP32 vector peaks (FP32
+    # without packed instructions runs at the FP64 rate, 78.6 TFLOP/s)
+    svd_roof = _roofline(("fp64 + fp32 vector", FP64_PEAK_TFS), f64 + f32, tsvd, "gesvd_kernel<float,3,true>",
+                         "flops executed: FP64 Householder QR of the 96x48 A^T, the measured FP32 then FP64 Jacobi "
+                         "sweeps on the 48x48 R^T (18*48 flops per pair, all 1128 pairs counted), the FP64 step "
+                         "between them (5*48^3), FP64 U = Q diag(U_R, I); the peak is FP64's, which non-packed FP32 "
+                         "arithmetic shares")
+    svd_roof["fp64_flops"] = f64
+    svd_roof["fp32_flops"] = f32
+    svd_roof["sweeps_fp32_mean"] = float(s32.mean())
+    svd_roof["sweeps_fp64_max"] = int(s64.max())
+    svd_roof["sweeps_fp64_mean"] = float(s64.mean())
+    svd_roof["all_fp64_sweeps"] = {"gesvd_ms": tsvd_64, "sweeps_max": int((sw_64 & 0xFFFF).max()),
+                                   "sweeps_mean": float((sw_64 & 0xFFFF).double().mean()),
+                                   "speedup_of_fp32_first": tsvd_64 / tsvd}
     reps = 1000
 
     def capture(calls):
@@ -635,7 +660,8 @@ def bench_svd_nullspace(a):
     res = np.linalg.norm(np.einsum("kij,kj->ki", Am, z), axis=1)
     rel = float(np.max(res / np.linalg.norm(x[:, 0, :].astype(np.float64), axis=1)))
     alg = k * (n * n * 4 + 2 * n * 4)
-    print(f"[bench] svd+nullspace build {tb:.3f} ms (svd {tsvd:.3f} ms, sweeps {int(sw.max())}, "
+    print(f"[bench] svd+nullspace build {tb:.3f} ms (svd {tsvd:.3f} ms, sweeps FP32 {s32.mean():.1f} + FP64 "
+          f"{s64.mean():.1f}; all-FP64 sweeps {tsvd_64:.3f} ms), "
           f"{svd_roof['achieved']:.2f} TFLOP/s); project {per_call_ms * 1e3:.2f} us/call in graph "
           f"({per_call_cycled_ms * 1e3:.2f} us with P cycled over {copies} copies), {ev * 1e3:.2f} us eager; "
           f"max |A P x|/|x| {rel:.2e}", file=sys.stderr, flush=True)
@@ -678,8 +704,8 @@ def bench_svd_nullspace(a):
                               "1000 graph-captured project calls per step", "k": k, "m": m, "n": n,
                   "calls_per_step": reps},
                  roof, cpu,
-                 build={"nullspace_build_ms": tb, "gesvd_ms": tsvd, "max_sweeps": int(sw.max()),
-                        "gesvd_with_vt_ms": tsvd_full, "max_sweeps_with_vt": int(sw_full.max()),
+                 build={"nullspace_build_ms": tb, "gesvd_ms": tsvd,
+                        "gesvd_with_vt_ms": tsvd_full, "max_sweeps_with_vt": int((sw_full & 0xFFFF).max()),
                         "ranks": sorted(set(rank.cpu().tolist())), "info_nonzero": int((info != 0).sum()),
                         "gesvd_roofline": svd_roof, "nullspace_builds_per_s": k / (tb * 1e-3)},
                  accuracy={"max_rel_A_Px": rel}, project_eager_event_ms=ev,

@@ -376,11 +376,13 @@ class RaggedLeastSquares:
 
 # ------------------------------------------------------------------ SVD --
 def gesvd_batched(A: torch.Tensor, compute_u: bool = False, sweeps: bool = False, compute_v: bool = True):
-    """SVD of every tall m x n matrix of A (tensor layout (k, n, m), m >= n,
-    n <= 64).  Returns (S (k, 1, n), U (k, m, m) or None, VT (k, n, n) or
-    None, info (k,)[, sweeps (k,)]).  A is not modified.  compute_v=False
-    skips the right singular vectors (jobvt 'N'): the sweeps then run on
-    R^T, giving the same S and column spaces of U (signs may differ)."""
+    """SVD of every tall m x n matrix of A (tensor layout (k, n, m), m >= n).
+    Returns (S (k, 1, n), U (k, m, m) or None, VT (k, n, n) or None, info
+    (k,)[, sweeps (k,)]).  A is not modified.  compute_v=False skips the
+    right singular vectors (jobvt 'N'): the sweeps then run on R^T, giving
+    the same S and column spaces of U (signs may differ).  sweeps holds the
+    FP64 sweep count in bits 0..15 and the FP32 one in bits 16..31
+    (sweep_counts splits them)."""
     _check_dev(A, "A")
     k, n, m = A.shape
     dev, dt = A.device, A.dtype
@@ -399,6 +401,17 @@ def gesvd_batched(A: torch.Tensor, compute_u: bool = False, sweeps: bool = False
         _ptr(VT) if VT is not None else None, n, n * n,
         _ptr(work) if work is not None else None, _ptr(info), _ptr(sw), k, stream_handle()), "gesvd_batched")
     return (S, U, VT, info, sw) if sweeps else (S, U, VT, info)
+
+
+def sweep_counts(sw: torch.Tensor):
+    """(FP64 sweeps, FP32 sweeps) per matrix from gesvd_batched's sweeps."""
+    return sw & 0xFFFF, sw >> 16
+
+
+def set_fp32_sweeps(enable: bool) -> bool:
+    """Whether float SVDs start their Jacobi sweeps in FP32 (the default) or
+    run them all in FP64; returns the previous setting (process-wide)."""
+    return bool(lib().gputils_gesvd_set_fp32_sweeps(1 if enable else 0))
 
 
 def svd_rank_batched(S: torch.Tensor, eps: float = 1e-6) -> torch.Tensor:

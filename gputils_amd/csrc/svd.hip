@@ -35,7 +35,9 @@
 // scalars.  All but kWide hold each Jacobi pair's column entries in
 // registers (n <= 64); kWide loops over them.
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
+#include <type_traits>
 
 #include "common.hpp"
 #include "householder.hpp"
@@ -122,7 +124,9 @@ __host__ __device__ inline Layout layout(int m, int n, int mode) {
 // t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)) with zeta = (be - al) / (2 ga),
 // multiplied through by 2 |ga|: one square root and one division, both
 // from the hardware estimates with one Newton step each (t only sets the
-// angle; cs, sn form a rotation for any t, accurate to a few ulp).
+// angle).  cs = 1 / sqrt(1 + t^2) takes two Newton steps, so that
+// cs^2 + sn^2 = 1 to FP64 rounding and the rotation itself loses no
+// orthogonality above the convergence tolerance n eps.
 __device__ __forceinline__ void jacobi_angle(double al, double be, double ga, double &cs, double &sn) {
     const double d = be - al, g2 = 2.0 * fabs(ga);
     const double sg = (d == 0.0 || ((d > 0.0) == (ga > 0.0))) ? 1.0 : -1.0;
@@ -133,11 +137,25 @@ __device__ __forceinline__ void jacobi_angle(double al, double be, double ga, do
     double inv = __builtin_amdgcn_rcp(den);
     inv = inv * fma(-den, inv, 2.0);
     const double t = sg * g2 * inv;
-    // cs = 1 / sqrt(1 + t^2): hardware estimate and one Newton step
-    // (relative error ~ the estimate's squared)
     const double t2p1 = fma(t, t, 1.0);
     cs = __builtin_amdgcn_rsq(t2p1);
     cs = cs * fma(-0.5 * t2p1, cs * cs, 1.5);
+    cs = cs * fma(-0.5 * t2p1, cs * cs, 1.5);
+    sn = cs * t;
+}
+
+// The same rotation in FP32 for the first sweeps of float data: the
+// hardware square root and reciprocal estimates (1 ulp) without Newton
+// steps.  The caller scales al, be, ga to max(al, be) = 1 and calls it only
+// when ga^2 > tol^2 al be, so that h >= min(1/4, 2 tol^2) is a normal
+// number (|d| >= 1/2 unless both sums exceed 1/2).
+__device__ __forceinline__ void jacobi_angle(float al, float be, float ga, float &cs, float &sn) {
+    const float d = be - al, g2 = 2.0f * fabsf(ga);
+    const float sg = (d == 0.0f || ((d > 0.0f) == (ga > 0.0f))) ? 1.0f : -1.0f;
+    const float h = fmaf(d, d, g2 * g2);
+    const float den = fmaf(h, __builtin_amdgcn_rsqf(h), fabsf(d));  // |d| + sqrt(h)
+    const float t = sg * g2 * __builtin_amdgcn_rcpf(den);
+    cs = __builtin_amdgcn_rsqf(fmaf(t, t, 1.0f));
     sn = cs * t;
 }
 
@@ -161,6 +179,15 @@ __device__ __forceinline__ double group8_sum(double x) {
 }
 __device__ __forceinline__ double group16_sum(double x) {
     return dpp_plus<0x140>(group8_sum(x));  // row_mirror
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_plus(float x) {
+    return x + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float group8_sum(float x) {
+    x = dpp_plus<0xB1>(x);
+    x = dpp_plus<0x4E>(x);
+    return dpp_plus<0x141>(x);
 }
 
 // Pair i of round r in a round-robin tournament over n2 (even) players:
@@ -251,7 +278,223 @@ __device__ void complete_basis_strided(double *X, int n, int nz, int lane) {
     }
 }
 
-template <typename T, int MODE>
+// Jacobi sweeps in the arithmetic of R on W (n x n, column-major, ld n),
+// the rotations accumulated in X, until a sweep rotates no pair or
+// max_sweeps have run; returns whether the last sweep rotated nothing.
+// Pair (p, q) is rotated when |w_p . w_q| > tol |w_p| |w_q| (squared: no
+// square roots on a round's critical path).  Parallel tournament ordering:
+// n2/2 disjoint pairs per round, 8 lanes per pair, one barrier per round.
+// All threads of the block call it.
+// KPER: column entries per lane (n <= 8 KPER) in the register path.
+template <typename R, int MODE, int KPER>
+__device__ __forceinline__ bool jacobi_sweeps_k(R *W, R *X, int n, R tol2, int max_sweeps, int &sweeps) {
+    const int tid = threadIdx.x;
+    const int n2 = n + (n & 1), n1 = n2 - 1, npairs = n2 >> 1;
+    const int grp = tid >> 3, gl = tid & 7;
+    sweeps = 0;
+    bool converged = (n1 == 0);
+    while (!converged && sweeps < max_sweeps) {
+        ++sweeps;
+        int rot = 0;
+        // players of pair grp in round 0 (tournament_pair), advanced by one
+        // place per round: no division on the round's path
+        int pa = (grp == 0) ? 0 : 1 + grp, pb = (grp == 0) ? 1 : 1 + n1 - grp;
+        for (int r = 0; r < n1; ++r) {
+            if constexpr (MODE == kWide) {
+                for (int pr = grp; pr < npairs; pr += kThreads / 8) {
+                    int p, q;
+                    tournament_pair(pr, r, n1, p, q);
+                    if (q < n && jacobi_pair_strided(W + (int64_t)p * n, W + (int64_t)q * n, X + (int64_t)p * n,
+                                                     X + (int64_t)q * n, n, gl, tol2))
+                        rot = 1;
+                }
+            } else if (grp < npairs) {
+                const int p = pa < pb ? pa : pb, q = pa < pb ? pb : pa;
+                if (grp != 0) pa = (pa == n1) ? 1 : pa + 1;
+                pb = (pb == n1) ? 1 : pb + 1;
+                if (q < n) {
+                    R *wp = W + p * n, *wq = W + q * n;
+                    R *vp = X + p * n, *vq = X + q * n;
+                    // the lane's entries i = gl + 8 u of both columns, loaded
+                    // together (indices clamped into the column, zero past n,
+                    // which adds nothing below) and kept for the rotation.
+                    // FP32 loads the X columns with them (registers to
+                    // spare), FP64 only once the pair is known to rotate.
+                    constexpr int kPer = KPER;
+                    constexpr bool kEarlyX = sizeof(R) == 4;
+                    R xw[kPer], yw[kPer], xv[kPer], yv[kPer];
+#pragma unroll
+                    for (int u = 0; u < kPer; ++u) {
+                        const int i = gl + 8 * u, ic = i < n ? i : n - 1;
+                        xw[u] = wp[ic];
+                        yw[u] = wq[ic];
+                        if (kEarlyX) {
+                            xv[u] = vp[ic];
+                            yv[u] = vq[ic];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < kPer; ++u) {
+                        if (gl + 8 * u >= n) xw[u] = yw[u] = R(0);
+                    }
+                    R al = R(0), be = R(0), ga = R(0);
+#pragma unroll
+                    for (int u = 0; u < kPer; ++u) {
+                        al = fma(xw[u], xw[u], al);
+                        be = fma(yw[u], yw[u], be);
+                        ga = fma(xw[u], yw[u], ga);
+                    }
+                    al = group8_sum(al);
+                    be = group8_sum(be);
+                    ga = group8_sum(ga);
+                    if constexpr (sizeof(R) == 4) {
+                        // scale the pair's sums to max(al, be) = 1 (the angle
+                        // and the test are scale-free), so that neither the
+                        // test nor h in the angle underflows for the tiny
+                        // columns of a rank-deficient matrix; pairs of two
+                        // columns below the FP32 normal range are left to FP64
+                        const R mab = al > be ? al : be;
+                        const R sc = mab >= FLT_MIN ? __builtin_amdgcn_rcpf(mab) : R(0);
+                        al *= sc;
+                        be *= sc;
+                        ga *= sc;
+                    }
+                    if (ga * ga > tol2 * al * be) {
+                        R cs, sn;
+                        jacobi_angle(al, be, ga, cs, sn);
+                        if (!kEarlyX) {
+#pragma unroll
+                            for (int u = 0; u < kPer; ++u) {
+                                const int i = gl + 8 * u, ic = i < n ? i : n - 1;
+                                xv[u] = vp[ic];
+                                yv[u] = vq[ic];
+                            }
+                        }
+#pragma unroll
+                        for (int u = 0; u < kPer; ++u) {
+                            const int i = gl + 8 * u;
+                            if (i < n) {
+                                wp[i] = cs * xw[u] - sn * yw[u];
+                                wq[i] = sn * xw[u] + cs * yw[u];
+                                vp[i] = cs * xv[u] - sn * yv[u];
+                                vq[i] = sn * xv[u] + cs * yv[u];
+                            }
+                        }
+                        rot = 1;
+                    }
+                }
+            }
+            if (r + 1 < n1) __syncthreads();
+        }
+        converged = !__syncthreads_or(rot);
+    }
+    return converged;
+}
+
+template <typename R, int MODE>
+__device__ __forceinline__ bool jacobi_sweeps(R *W, R *X, int n, R tol2, int max_sweeps, int &sweeps) {
+    if constexpr (MODE != kWide) {
+        if (n <= 32) return jacobi_sweeps_k<R, MODE, 4>(W, X, n, tol2, max_sweeps, sweeps);
+        if (n <= 48) return jacobi_sweeps_k<R, MODE, 6>(W, X, n, tol2, max_sweeps, sweeps);
+    }
+    return jacobi_sweeps_k<R, MODE, kMaxN / 8>(W, X, n, tol2, max_sweeps, sweeps);
+}
+
+// FP32 first sweeps for float data (n <= 64 held in LDS): on entry W holds
+// the FP64 n x n matrix W0 to be swept and X (n x n doubles) is free; on
+// return W = W0 X and X holds the FP32 sweeps' rotations, made orthogonal
+// to FP64 level, ready for the FP64 sweeps.
+//   * W0 is scaled by a power of two to max |w| in [1, 2) and rounded to
+//     FP32 (W32) in the first half of X's space, X32 = I in the second;
+//     the FP32 sweeps run to the FP32 tolerance n u32 (rotations do not
+//     depend on the scale).
+//   * X32 is orthogonal to about n u32.  One Newton-Schulz step
+//     X = X32 - F X32 / 2 with F = X32 X32^T - I (FP64 sums; F ~ 1e-6 is
+//     kept in FP32 over W32, which adds ~1e-13) brings that to ~1e-11.
+//   * W = W0 X in FP64, so that W = W0 X holds to FP64 rounding: the FP64
+//     sweeps then start from an off-diagonal level of ~u32 and need two or
+//     three sweeps.
+// Lanes run over the row index i of each product, so one operand is read
+// along a column (consecutive addresses) and the other is the same for the
+// whole wave (broadcast).  Returns the FP32 sweep count (0 when max |W0| is
+// zero or not finite: the FP64 sweeps then start from W0 and X = I).
+template <int MODE>
+__device__ int fp32_first_sweeps(double *W, double *X, int n, bool lower, double *red) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nn = n * n;
+    double mx = 0.0;
+    for (int e = tid; e < nn; e += kThreads) mx = fmax(mx, fabs(W[e]));
+    mx = wave_max(mx);
+    if (lane == 0) red[wave] = mx;
+    __syncthreads();
+    mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    const bool run = mx > 0.0 && mx <= DBL_MAX;  // uniform
+    float *W32 = reinterpret_cast<float *>(X), *X32 = W32 + nn;
+    int s32 = 0;
+    if (run) {
+        const double sc = ldexp(1.0, -ilogb(mx));
+        for (int e = tid; e < nn; e += kThreads) {
+            W32[e] = (float)(W[e] * sc);
+            X32[e] = (e % n == e / n) ? 1.0f : 0.0f;
+        }
+        __syncthreads();
+        const float tol = (float)(n > 2 ? n : 2) * FLT_EPSILON;
+        jacobi_sweeps<float, MODE>(W32, X32, n, tol * tol, kMaxSweeps, s32);
+        float *F = W32;  // W32 is not needed any more
+        for (int e = tid; e < nn; e += kThreads) {
+            const int i = e % n, j = e / n;
+            double acc = (i == j) ? -1.0 : 0.0;
+            for (int k = 0; k < n; ++k) acc = fma((double)X32[i + k * n], (double)X32[j + k * n], acc);
+            F[e] = (float)acc;
+        }
+        __syncthreads();
+    }
+    constexpr int kR = (kMaxN * kMaxN + kThreads - 1) / kThreads;
+    double xr[kR];
+#pragma unroll
+    for (int t = 0; t < kR; ++t) {
+        const int e = tid + kThreads * t, i = e % n, j = e / n;
+        if (e < nn) {
+            if (run) {
+                double acc = (double)X32[e];
+                for (int k = 0; k < n; ++k) acc = fma(-0.5 * (double)W32[i + k * n], (double)X32[k + j * n], acc);
+                xr[t] = acc;
+            } else {
+                xr[t] = (i == j) ? 1.0 : 0.0;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < kR; ++t) {
+        const int e = tid + kThreads * t;
+        if (e < nn) X[e] = xr[t];
+    }
+    __syncthreads();
+    if (run) {
+        // W0 is triangular: lower (R^T) or upper (R)
+#pragma unroll
+        for (int t = 0; t < kR; ++t) {
+            const int e = tid + kThreads * t, i = e % n, j = e / n;
+            if (e < nn) {
+                const int k0 = lower ? 0 : i, k1 = lower ? i + 1 : n;
+                double acc = 0.0;
+                for (int k = k0; k < k1; ++k) acc = fma(W[i + k * n], X[k + j * n], acc);
+                xr[t] = acc;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < kR; ++t) {
+            const int e = tid + kThreads * t;
+            if (e < nn) W[e] = xr[t];
+        }
+        __syncthreads();
+    }
+    return s32;
+}
+
+template <typename T, int MODE, bool MIXED>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void gesvd_kernel(int m, int n, const T *__restrict__ A, int64_t lda,
                                                          int64_t strideA, T *S, int64_t strideS, T *U,
                                                          int64_t ldu, int64_t strideU, T *VT, int64_t ldvt,
@@ -324,88 +567,18 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     }
     __syncthreads();
 
-    // 2. one-sided Jacobi on W, V accumulated in X
-    const int n2 = n + (n & 1), n1 = n2 - 1, npairs = n2 >> 1;
-    const int grp = tid >> 3, gl = tid & 7;
-    const double tol = (double)(n > 2 ? n : 2) * DBL_EPSILON, tol2 = tol * tol;
-    int sweeps = 0;
-    bool converged = (n1 == 0);
-    while (!converged && sweeps < kMaxSweeps) {
-        ++sweeps;
-        int rot = 0;
-        for (int r = 0; r < n1; ++r) {
-            if constexpr (MODE == kWide) {
-                for (int pr = grp; pr < npairs; pr += kThreads / 8) {
-                    int p, q;
-                    tournament_pair(pr, r, n1, p, q);
-                    if (q < n && jacobi_pair_strided(W + (int64_t)p * n, W + (int64_t)q * n, X + (int64_t)p * n,
-                                                     X + (int64_t)q * n, n, gl, tol2))
-                        rot = 1;
-                }
-            } else if (grp < npairs) {
-                int p, q;
-                tournament_pair(grp, r, n1, p, q);
-                if (q < n) {
-                    double *wp = W + p * n, *wq = W + q * n;
-                    // the lane's entries i = gl + 8 u of both columns, loaded
-                    // together (indices clamped into the column, zero past n,
-                    // which adds nothing below) and kept for the rotation
-                    constexpr int kPer = kMaxN / 8;
-                    double xw[kPer], yw[kPer];
-#pragma unroll
-                    for (int u = 0; u < kPer; ++u) {
-                        const int i = gl + 8 * u, ic = i < n ? i : n - 1;
-                        xw[u] = wp[ic];
-                        yw[u] = wq[ic];
-                    }
-#pragma unroll
-                    for (int u = 0; u < kPer; ++u) {
-                        if (gl + 8 * u >= n) xw[u] = yw[u] = 0.0;
-                    }
-                    double al = 0.0, be = 0.0, ga = 0.0;
-#pragma unroll
-                    for (int u = 0; u < kPer; ++u) {
-                        al = fma(xw[u], xw[u], al);
-                        be = fma(yw[u], yw[u], be);
-                        ga = fma(xw[u], yw[u], ga);
-                    }
-                    al = group8_sum(al);
-                    be = group8_sum(be);
-                    ga = group8_sum(ga);
-                    // |ga| > tol sqrt(al be), squared: no square roots on the
-                    // round's critical path
-                    if (ga * ga > tol2 * al * be) {
-                        double cs, sn;
-                        jacobi_angle(al, be, ga, cs, sn);
-                        double *vp = X + p * n, *vq = X + q * n;
-                        double xv[kPer], yv[kPer];
-#pragma unroll
-                        for (int u = 0; u < kPer; ++u) {
-                            const int i = gl + 8 * u, ic = i < n ? i : n - 1;
-                            xv[u] = vp[ic];
-                            yv[u] = vq[ic];
-                        }
-#pragma unroll
-                        for (int u = 0; u < kPer; ++u) {
-                            const int i = gl + 8 * u;
-                            if (i < n) {
-                                wp[i] = cs * xw[u] - sn * yw[u];
-                                wq[i] = sn * xw[u] + cs * yw[u];
-                                vp[i] = cs * xv[u] - sn * yv[u];
-                                vq[i] = sn * xv[u] + cs * yv[u];
-                            }
-                        }
-                        rot = 1;
-                    }
-                }
-            }
-            if (r + 1 < n1) __syncthreads();
-        }
-        converged = !__syncthreads_or(rot);
+    // 2. one-sided Jacobi on W, V accumulated in X: for float data first in
+    // FP32 (MIXED), then in FP64 to the FP64 tolerance
+    int s32 = 0, s64 = 0;
+    if constexpr (MIXED) {
+        if (n >= 4) s32 = fp32_first_sweeps<MODE>(W, X, n, on_rt, sig);  // sig[0..3]: wave maxima
     }
+    const double tol = (double)(n > 2 ? n : 2) * DBL_EPSILON;
+    const bool converged = jacobi_sweeps<double, MODE>(W, X, n, tol * tol, kMaxSweeps, s64);
     __syncthreads();
 
     // 3. singular values, order, signs
+    const int grp = tid >> 3, gl = tid & 7;
     for (int j = grp; j < n; j += kThreads / 8) {
         double s = 0.0;
         for (int i = gl; i < n; i += 8) s = fma(W[i + j * n], W[i + j * n], s);
@@ -455,7 +628,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     }
     if (tid == 0) {
         info[b] = converged ? 0 : 1;
-        if (sweeps_out) sweeps_out[b] = sweeps;
+        if (sweeps_out) sweeps_out[b] = s64 | (s32 << 16);
     }
     if (U == nullptr) return;
     __syncthreads();
@@ -700,12 +873,21 @@ inline int gesvd_mode(int m, int n) {
     return kWide;
 }
 
+// FP32 first sweeps for float data (gputils_gesvd_set_fp32_sweeps)
+std::atomic<int> g_fp32_sweeps{1};
+
 template <typename T, int MODE>
 int launch_gesvd(int m, int n, const T *A, int64_t lda, int64_t strideA, T *S, int64_t strideS, T *U,
                  int64_t ldu, int64_t strideU, T *VT, int64_t ldvt, int64_t strideVT, double *work, int *info,
                  int *sweeps, int64_t batch, hipStream_t st) {
     const Layout L = layout(m, n, MODE);
-    auto kern = gesvd_kernel<T, MODE>;
+    // FP32 first sweeps: float data, W and V in LDS, and the sweeps on R^T
+    // (VT == NULL).  On R (VT wanted) the FP64 sweeps would have to
+    // orthogonalise the null cluster of a rank-deficient matrix from scratch
+    // after the FP32 ones, which costs more than it saves.
+    constexpr bool kCanMix = std::is_same<T, float>::value && MODE != kWide;
+    const bool mix = kCanMix && VT == nullptr && g_fp32_sweeps.load(std::memory_order_relaxed) != 0;
+    auto kern = mix ? gesvd_kernel<T, MODE, kCanMix> : gesvd_kernel<T, MODE, false>;
     if (L.bytes > 64 * 1024)
         GPUTILS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                         hipFuncAttributeMaxDynamicSharedMemorySize, L.bytes));
@@ -796,6 +978,10 @@ int nullspace_build(int n, const T *U, int64_t ldu, int64_t strideU, const unsig
 using namespace gputils;
 
 extern "C" {
+
+int gputils_gesvd_set_fp32_sweeps(int enable) {
+    return svd::g_fp32_sweeps.exchange(enable != 0 ? 1 : 0);
+}
 
 int64_t gputils_gesvd_workspace_bytes(int m, int n, int64_t batch) {
     if (m < n || n < 1 || batch < 0) return -1;

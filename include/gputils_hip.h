@@ -305,19 +305,27 @@ int gputils_graph_launch(gputils_graph_t graph, void *stream);
 int gputils_graph_destroy(gputils_graph_t graph);
 
 /* ------------------------------------------------------------ SVD / rank --- */
-/* A_i = U_i diag(S_i) VT_i for tall m x n matrices (m >= n, 1 <= n <= 64):
+/* A_i = U_i diag(S_i) VT_i for tall m x n matrices (m >= n >= 1):
  * replaces the per-matrix cusolverDn?gesvd loop of Svd::factorise
  * (tensor.cuh:1624-1676, jobu 'A' when U != NULL else 'N', jobvt 'A').
  * S_i: n values, descending.  VT_i: n x n (ld ldvt), row j = v_j^T with the
  * largest-magnitude entry of v_j negative.  U_i: m x m (ld ldu) or NULL.
- * A is read only.  FP64 arithmetic inside for both types.  info[i] = 0, or
- * 1 if the Jacobi sweeps did not converge; sweeps (may be NULL) receives the
- * sweep count.  work: gputils_gesvd_workspace_bytes(m, n, batch) bytes of
- * device memory (0 bytes, work may be NULL, when the matrix fits in LDS).
+ * A is read only.  Householder QR and the final Jacobi sweeps in FP64 for
+ * both types; for float data with 4 <= n <= 64 and VT == NULL the sweeps
+ * start in FP32 and FP64 sweeps finish them from the FP32 result
+ * (gputils_gesvd_set_fp32_sweeps turns that off).  info[i] = 0, or 1 if the FP64 sweeps did not converge;
+ * sweeps (may be NULL) receives the FP64 sweep count in bits 0..15 and the
+ * FP32 one in bits 16..31.  work: gputils_gesvd_workspace_bytes(m, n, batch)
+ * bytes of device memory (0 bytes, work may be NULL, when the matrix fits
+ * in LDS).
  * VT may be NULL (jobvt 'N', no reference counterpart; used by the nullspace
  * construction): the sweeps then run on R^T and U_R is the accumulated
  * rotation matrix, which gives the same S and the same column spaces of U
  * (column signs may differ from the VT != NULL result). */
+/* Whether float SVDs run their first sweeps in FP32 (1, the default) or
+ * all sweeps in FP64 (0); returns the previous setting.  Process-wide; no
+ * reference counterpart (a switch for measuring the two schemes). */
+int gputils_gesvd_set_fp32_sweeps(int enable);
 int64_t gputils_gesvd_workspace_bytes(int m, int n, int64_t batch);
 int gputils_sgesvd_batched(int m, int n, const float *A, int64_t lda, int64_t strideA, float *S, int64_t strideS,
                            float *U, int64_t ldu, int64_t strideU, float *VT, int64_t ldvt, int64_t strideVT,

@@ -46,7 +46,8 @@ def _msg():
     ("gputils_dpotrf_batched", (4, None, 4, 16, None, 1, None), -2),   # A NULL
     ("gputils_dpotrs_batched", (4, -1, None, 4, 16, None, 4, 4, None, 1, None), -2),
     ("gputils_dpotrs_batched", (4, 1, None, 4, 16, None, 3, 4, None, 1, None), -7),
-    ("gputils_dposv_batched", (129, 1, None, 129, 0, None, 129, 0, None, 1, None), -1),
+    ("gputils_dposv_batched", (-1, 1, None, 1, 0, None, 1, 0, None, 1, None), -1),
+    ("gputils_dposv_batched", (129, 1, None, 129, 0, None, 129, 0, None, 1, None), -3),  # any n; A NULL
     ("gputils_dposv_mixed_batched", (8, 1, None, 8, 64, None, 8, 8, None, 8, 8, 0.0, 3, None, None, None, 1, None), -12),
     ("gputils_dposv_mixed_batched", (8, 1, None, 8, 64, None, 8, 8, None, 8, 8, 1e-12, -1, None, None, None, 1, None), -13),
     ("gputils_sgemm_strided_batched", (2, 2, 2, 1.0, None, 1, 4, None, 2, 4, 0.0, None, 2, 4, 1, 0, None), -6),

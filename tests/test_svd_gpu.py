@@ -99,7 +99,9 @@ def test_svd_matches_oracle(oracle, m, n, dtype):
         Pc, Po = U[b][:, n:] @ U[b][:, n:].T, Uo[:, n:] @ Uo[:, n:].T
         assert np.allclose(Pc, Po, atol=tol * 100)
         assert np.linalg.norm(U[b][:, :n] * S[b] @ VT[b] - A[b]) <= tol * 10 * smax * np.sqrt(m * n)
-    assert sw.max() <= 20
+    s64, s32 = sw & 0xFFFF, sw >> 16
+    assert s64.max() <= 20
+    assert s32.max() == 0  # with VT the sweeps are all FP64
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
@@ -178,6 +180,51 @@ def test_svd_without_vt_same_values_and_subspaces(m, n, dtype):
             P1 = U1[b][:, lo:hi] @ U1[b][:, lo:hi].T
             P2 = U2[b][:, lo:hi] @ U2[b][:, lo:hi].T
             assert np.allclose(P1, P2, atol=1e3 * tol), (b, lo, hi, np.abs(P1 - P2).max())
+
+
+@pytest.mark.parametrize("m,n,rk", [(96, 48, 48), (96, 48, 40), (64, 64, 64), (200, 50, 50), (40, 12, 5),
+                                    (10, 4, 4)])
+def test_fp32_first_sweeps_match_fp64_sweeps(oracle, m, n, rk):
+    """Float data without VT (the nullspace builds' SVD): FP32 sweeps, then
+    FP64 sweeps from their result, against all-FP64 sweeps
+    (gputils_gesvd_set_fp32_sweeps(0)) and the FP64 oracle: the same
+    singular values to FP32 rounding (2e-6 sigma_max), U orthogonal to FP32
+    rounding, the same subspaces for the nonzero and the zero singular
+    values, and at most four FP64 sweeps (fewer than all-FP64 takes).  With
+    VT the sweeps stay FP64: bit-identical to the switch turned off."""
+    k = 4
+    r = np.random.default_rng(7 * m + n + rk)
+    A = np.einsum("kir,krj->kij", r.standard_normal((k, m, rk)), r.standard_normal((k, rk, n)))
+    A = A.astype(np.float32).astype(np.float64)
+    tA = tensor_of(A, torch.float32)
+    runs = {}
+    for mixed in (False, True):
+        prev = api.set_fp32_sweeps(mixed)
+        try:
+            runs[mixed] = [api.gesvd_batched(tA, compute_u=True, sweeps=True, compute_v=cv) for cv in (False, True)]
+        finally:
+            api.set_fp32_sweeps(prev)
+    assert api.set_fp32_sweeps(True) is True  # the default was restored
+    (S0, U0, _, info0, sw0), (Sv0, Uv0, VT0, _, _) = runs[False]
+    (S1, U1, _, info1, sw1), (Sv1, Uv1, VT1, _, swv1) = runs[True]
+    assert torch.equal(Sv0, Sv1) and torch.equal(Uv0, Uv1) and torch.equal(VT0, VT1)
+    assert int(api.sweep_counts(swv1)[1].max()) == 0
+    assert info0.tolist() == [0] * k and info1.tolist() == [0] * k
+    (a64, a32), (b64, b32) = api.sweep_counts(sw0), api.sweep_counts(sw1)
+    assert int(a32.max()) == 0 and int(b32.min()) >= 1
+    assert int(b64.max()) <= 4 and int(b64.max()) < int(a64.min()), (a64, b64, b32)
+    S0, S1 = S0.cpu().double().numpy()[:, 0, :], S1.cpu().double().numpy()[:, 0, :]
+    U0, U1 = api.to_matrices(U0).cpu().double().numpy(), api.to_matrices(U1).cpu().double().numpy()
+    for b in range(k):
+        So = oracle.gesvd(A[b].copy())[0]
+        smax = So[0]
+        assert np.max(np.abs(S1[b] - So)) <= 2e-6 * smax
+        assert np.max(np.abs(S1[b] - S0[b])) <= 2e-6 * smax
+        assert np.allclose(U1[b].T @ U1[b], np.eye(m), atol=2e-5)
+        for lo, hi in ((0, rk), (rk, m)):
+            P0 = U0[b][:, lo:hi] @ U0[b][:, lo:hi].T
+            P1 = U1[b][:, lo:hi] @ U1[b][:, lo:hi].T
+            assert np.abs(P1 - P0).max() <= 1e-4, (b, lo, hi)
 
 
 def test_svd_argument_checks():
