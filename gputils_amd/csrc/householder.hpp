@@ -148,6 +148,83 @@ __device__ __forceinline__ void apply_reflector_col(T *A, int64_t lda, int m, in
     if (lane == 0) x[j] -= tw;
 }
 
+// x + (x of the lane that DPP control CTRL selects), within 16-lane rows
+template <int CTRL>
+__device__ __forceinline__ float row16_plus(float x) {
+    return x + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double row16_plus(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return x + __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over each aligned group of 16 lanes, in every lane of the group:
+// quad permutations [1,0,3,2] and [2,3,0,1], then the row half mirror and
+// the row mirror.  Register moves only.
+template <typename T>
+__device__ __forceinline__ T row16_sum(T x) {
+    x = row16_plus<0xB1>(x);   // quad_perm [1,0,3,2]
+    x = row16_plus<0x4E>(x);   // quad_perm [2,3,0,1]
+    x = row16_plus<0x141>(x);  // row_half_mirror
+    return row16_plus<0x140>(x);  // row_mirror
+}
+
+// x_c <- (I - tau v v^T) x_c for the columns c = c0, c0 + NW, ... < n (the
+// columns one of NW wavefronts owns), v = [1; A[j+1:m, j]], for
+// m - j - 1 <= 16 KR: each 16-lane group of the wavefront takes one column
+// (four at a time), lane l of a group rows j + 1 + l + 16 k (k < KR, v held
+// in registers for all the columns), and reduces its dot product within
+// its 16 lanes.  Four columns then cost one set of row loads and one
+// four-step reduction, where the 64-lane form below pays a six-step
+// reduction per column.
+template <typename T, typename DOT, int NW, int KR>
+__device__ __forceinline__ void apply_reflector_cols16(T *A, int64_t lda, int m, int j, T tau, int c0, int n) {
+    const int lane = threadIdx.x & 63, g = lane >> 4, l16 = lane & 15;
+    const T *v = A + (int64_t)j * lda;
+    const int kr = (m - j - 1 + 15) >> 4;  // row groups in use (uniform)
+    T vr[KR];
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+        const int i = j + 1 + l16 + 16 * k;
+        vr[k] = T(0);
+        if (k < kr) {
+            const T e = v[i < m ? i : m - 1];  // clamped: every load is valid
+            vr[k] = (i < m) ? e : T(0);
+        }
+    }
+    for (int cq = c0; cq < n; cq += 4 * NW) {
+        const int c = cq + NW * g;
+        const bool ok = c < n;
+        T *x = A + (int64_t)(ok ? c : cq) * lda;
+        T xr[KR];
+        DOT w = DOT(0);
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+            xr[k] = T(0);
+            if (k < kr) {
+                const int i = j + 1 + l16 + 16 * k;
+                xr[k] = x[i < m ? i : m - 1];
+                w += (DOT)vr[k] * (DOT)xr[k];  // vr is zero past m
+            }
+        }
+        const T xj = x[j];
+        w = row16_sum(w) + (DOT)xj;
+        const T tw = ok ? (T)((DOT)tau * w) : T(0);
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+            if (k < kr) {
+                const int i = j + 1 + l16 + 16 * k;
+                if (ok && i < m) x[i] = xr[k] - tw * vr[k];
+            }
+        }
+        if (ok && l16 == 0) x[j] = xj - tw;
+    }
+    // the caller's barrier orders these writes before other wavefronts' reads
+}
+
 // x_c <- (I - tau v v^T) x_c for the columns c = c0, c0 + NW, ... < n (the
 // columns one of NW wavefronts owns), v = [1; A[j+1:m, j]], four columns at a
 // time: their dot products are accumulated together and their reductions
@@ -200,8 +277,10 @@ __device__ __forceinline__ void apply_reflector_cols(T *A, int64_t lda, int m, i
 // ntot-1 (stored after the factorised ones) receive every reflector
 // without being factorised: a right-hand side stored there ends as Q^T b.
 // NW (a power of two) wavefronts: column c is owned by wavefront c % NW.
-// Each column's arithmetic is the same for every NW.
-template <typename T, typename ACC, typename DOT = ACC, int NW = kWaves>
+// Each column's arithmetic is the same for every NW.  KR16 > 0 (requires
+// m <= 16 KR16 + 1) applies the reflectors with 16-lane groups
+// (apply_reflector_cols16).
+template <typename T, typename ACC, typename DOT = ACC, int NW = kWaves, int KR16 = 0>
 __device__ __forceinline__ void hqr_factor(T *A, int64_t lda, int m, int n, T *tau, T *s_tau, int ntot = 0) {
     static_assert(NW >= 1 && (NW & (NW - 1)) == 0, "NW must be a power of two");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -239,8 +318,11 @@ __device__ __forceinline__ void hqr_factor(T *A, int64_t lda, int m, int n, T *t
                 }
             }
         }
-        if (t != T(0))
-            apply_reflector_cols<T, DOT, NW>(A, lda, m, j, t, j + 2 + ((wave - (j + 2)) & (NW - 1)), ntot);
+        if (t != T(0)) {
+            const int c0 = j + 2 + ((wave - (j + 2)) & (NW - 1));
+            if constexpr (KR16 > 0) apply_reflector_cols16<T, DOT, NW, KR16>(A, lda, m, j, t, c0, ntot);
+            else apply_reflector_cols<T, DOT, NW>(A, lda, m, j, t, c0, ntot);
+        }
         __syncthreads();
     }
 }

@@ -524,7 +524,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
         G[i + (int64_t)j * m] = (double)a[i + (int64_t)j * lda];
     }
     __syncthreads();
-    hh::hqr_factor<double, double>(G, m, m, n, tau, s_tau);
+    if (m <= 16 * 8 + 1) hh::hqr_factor<double, double, double, hh::kWaves, 8>(G, m, m, n, tau, s_tau);
+    else hh::hqr_factor<double, double>(G, m, m, n, tau, s_tau);
     // VT == NULL (no right singular vectors): the sweeps run on W = R^T, so
     // that the accumulated rotations X are the left singular vectors of R
     // (R^T Z = V_R Sigma  =>  R = Z Sigma V_R^T): no completion is needed for
