@@ -180,14 +180,36 @@ __device__ __forceinline__ T row16_sum(T x) {
 // its 16 lanes.  Four columns then cost one set of row loads and one
 // four-step reduction, where the 64-lane form below pays a six-step
 // reduction per column.
+// KR < 0: any m, nothing held in registers (two passes over the rows, v
+// read with each; for kernels that cannot spare the registers).
 template <typename T, typename DOT, int NW, int KR>
 __device__ __forceinline__ void apply_reflector_cols16(T *A, int64_t lda, int m, int j, T tau, int c0, int n) {
     const int lane = threadIdx.x & 63, g = lane >> 4, l16 = lane & 15;
     const T *v = A + (int64_t)j * lda;
+    if constexpr (KR < 0) {
+        for (int cq = c0; cq < n; cq += 4 * NW) {
+            const int c = cq + NW * g;
+            const bool ok = c < n;
+            T *x = A + (int64_t)(ok ? c : cq) * lda;
+            DOT w = DOT(0);
+#pragma unroll 4
+            for (int i = j + 1 + l16; i < m; i += 16) w += (DOT)v[i] * (DOT)x[i];
+            const T xj = x[j];
+            w = row16_sum(w) + (DOT)xj;
+            const T tw = ok ? (T)((DOT)tau * w) : T(0);
+            if (ok) {
+#pragma unroll 4
+                for (int i = j + 1 + l16; i < m; i += 16) x[i] -= tw * v[i];
+                if (l16 == 0) x[j] = xj - tw;
+            }
+        }
+        return;
+    }
+    constexpr int KRH = KR > 0 ? KR : 1;
     const int kr = (m - j - 1 + 15) >> 4;  // row groups in use (uniform)
-    T vr[KR];
+    T vr[KRH];
 #pragma unroll
-    for (int k = 0; k < KR; ++k) {
+    for (int k = 0; k < KRH; ++k) {
         const int i = j + 1 + l16 + 16 * k;
         vr[k] = T(0);
         if (k < kr) {
@@ -199,10 +221,10 @@ __device__ __forceinline__ void apply_reflector_cols16(T *A, int64_t lda, int m,
         const int c = cq + NW * g;
         const bool ok = c < n;
         T *x = A + (int64_t)(ok ? c : cq) * lda;
-        T xr[KR];
+        T xr[KRH];
         DOT w = DOT(0);
 #pragma unroll
-        for (int k = 0; k < KR; ++k) {
+        for (int k = 0; k < KRH; ++k) {
             xr[k] = T(0);
             if (k < kr) {
                 const int i = j + 1 + l16 + 16 * k;
@@ -214,7 +236,7 @@ __device__ __forceinline__ void apply_reflector_cols16(T *A, int64_t lda, int m,
         w = row16_sum(w) + (DOT)xj;
         const T tw = ok ? (T)((DOT)tau * w) : T(0);
 #pragma unroll
-        for (int k = 0; k < KR; ++k) {
+        for (int k = 0; k < KRH; ++k) {
             if (k < kr) {
                 const int i = j + 1 + l16 + 16 * k;
                 if (ok && i < m) x[i] = xr[k] - tw * vr[k];
@@ -277,9 +299,9 @@ __device__ __forceinline__ void apply_reflector_cols(T *A, int64_t lda, int m, i
 // ntot-1 (stored after the factorised ones) receive every reflector
 // without being factorised: a right-hand side stored there ends as Q^T b.
 // NW (a power of two) wavefronts: column c is owned by wavefront c % NW.
-// Each column's arithmetic is the same for every NW.  KR16 > 0 (requires
-// m <= 16 KR16 + 1) applies the reflectors with 16-lane groups
-// (apply_reflector_cols16).
+// Each column's arithmetic is the same for every NW.  KR16 != 0 applies the
+// reflectors with 16-lane groups (apply_reflector_cols16; KR16 > 0 requires
+// m <= 16 KR16 + 1).
 template <typename T, typename ACC, typename DOT = ACC, int NW = kWaves, int KR16 = 0>
 __device__ __forceinline__ void hqr_factor(T *A, int64_t lda, int m, int n, T *tau, T *s_tau, int ntot = 0) {
     static_assert(NW >= 1 && (NW & (NW - 1)) == 0, "NW must be a power of two");
@@ -320,7 +342,7 @@ __device__ __forceinline__ void hqr_factor(T *A, int64_t lda, int m, int n, T *t
         }
         if (t != T(0)) {
             const int c0 = j + 2 + ((wave - (j + 2)) & (NW - 1));
-            if constexpr (KR16 > 0) apply_reflector_cols16<T, DOT, NW, KR16>(A, lda, m, j, t, c0, ntot);
+            if constexpr (KR16 != 0) apply_reflector_cols16<T, DOT, NW, KR16>(A, lda, m, j, t, c0, ntot);
             else apply_reflector_cols<T, DOT, NW>(A, lda, m, j, t, c0, ntot);
         }
         __syncthreads();

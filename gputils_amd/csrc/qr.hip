@@ -515,7 +515,8 @@ __global__ __launch_bounds__(NW * kWave) void gels_ragged_kernel(const RaggedIte
                 qb[i] = (float)v;
             }
             __syncthreads();
-            hqr_factor<float, double, float, NW>(S, m, m, n, tau, s_tau, n + 1);  // FP64 norms, FP32 dots
+            // FP64 norms, FP32 dots; trailing columns updated by 16-lane groups
+            hqr_factor<float, double, float, NW, -1>(S, m, m, n, tau, s_tau, n + 1);
         }
         if (wave == 0) trsv_upper_regs(S, m, n, qb);  // x0 = R^{-1} (Q^T b)[0:n]
         __syncthreads();
