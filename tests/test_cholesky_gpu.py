@@ -174,6 +174,32 @@ def test_fused_posv_matches_oracle(oracle, n, nrhs):
         assert np.max(np.abs(X[b].T - ref)) <= 1e-11 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("n", [40, 128, 140])
+def test_fp64_posv_tiny_and_huge_pivots(oracle, n):
+    """The blocked FP64 kernels form each pivot's square root and its
+    reciprocal from the hardware 1/sqrt estimate; pivots below 2^-767 take
+    a pre-scaled path.  A scaled by 2^-800 (every pivot below that bound)
+    and by 2^+700 gives, up to FP64 rounding, the factor of A scaled by
+    2^-400 / 2^+350 and the same solution (exact powers of two, no
+    underflow or overflow on the way)."""
+    k = 3
+    A = spd_batch(k, n, seed=300 + n)
+    B = np.random.default_rng(n + 1).standard_normal((k, 2, n))
+    Lo = [oracle.potrf(A[b])[0] for b in range(k)]
+    Xo = [oracle.potrs(Lo[b], B[b].T) for b in range(k)]
+    for e in (-800, 700):
+        sc = 2.0 ** e
+        tA, tB = dev(A * sc), dev(B * sc)
+        info = api.posv_batched(tA, tB)
+        assert info.cpu().tolist() == [0] * k
+        L = api.to_matrices(tA).cpu().numpy()
+        X = tB.cpu().numpy()
+        for b in range(k):
+            Ls = np.tril(L[b]) * 2.0 ** (-e // 2)
+            assert np.max(np.abs(Ls - np.tril(Lo[b]))) <= 1e-13 * np.abs(Lo[b]).max(), (e, b)
+            assert np.max(np.abs(X[b].T - Xo[b])) <= 1e-11 * np.abs(Xo[b]).max(), (e, b)
+
+
 @pytest.mark.parametrize("n", [40, 140])
 def test_fused_posv_failure_reports_info(oracle, n):
     """A non-SPD system reports the oracle's info (blocked kernel for
