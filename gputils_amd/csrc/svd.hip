@@ -7,15 +7,20 @@
 // (tensor.cuh:2046-2079).
 //
 // One 256-thread workgroup per matrix, FP64 arithmetic for both float and
-// double data:
+// double data except the first Jacobi sweeps of float data without VT
+// (step 2):
 //   1. G = A (m x n, m >= n) is staged in LDS and factorised G = Q [R; 0]
 //      (Householder, LAPACK conventions).  LAPACK dgesvd takes the same
 //      QR-first path for m >= 1.6 n, so Q[:, n:m] is the completion of U it
 //      returns there.
-//   2. One-sided (Hestenes) Jacobi on the columns of W = R, V accumulated.
-//      Parallel tournament ordering: n/2 disjoint column pairs per round,
-//      8 lanes per pair, one barrier per round; sweeps repeat until no pair
-//      needs a rotation, |w_p . w_q| <= n eps ||w_p|| ||w_q||.
+//   2. One-sided (Hestenes) Jacobi on the columns of W = R (or R^T when VT
+//      is not wanted), V accumulated.  Parallel tournament ordering: n/2
+//      disjoint column pairs per round, 8 lanes per pair, one barrier per
+//      round; sweeps repeat until no pair needs a rotation,
+//      |w_p . w_q| <= n eps ||w_p|| ||w_q||.  Float data without VT
+//      (n <= 64) sweeps first in FP32 to the FP32 tolerance, then one
+//      Newton-Schulz step and W = W0 V in FP64 (fp32_first_sweeps), and
+//      the FP64 sweeps finish from there.
 //   3. sigma_j = ||w_j||, sorted descending; for nonzero sigma_j, v_j is
 //      signed so that its largest-magnitude entry is negative and
 //      u_j = w_j / sigma_j.  For a numerically zero sigma_j
