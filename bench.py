@@ -622,49 +622,63 @@ def bench_svd_nullspace(a):
                                    "speedup_of_fp32_first": tsvd_64 / tsvd}
     reps = 1000
 
-    def capture(calls):
-        """One HIP graph of `reps` project calls; calls(i) -> (P_i, b)."""
+    def capture(call):
+        """One HIP graph of `reps` project calls; call(i) makes call i."""
         st = torch.cuda.Stream()
         st.wait_stream(torch.cuda.current_stream())
         gr = torch.cuda.CUDAGraph()
         with torch.cuda.stream(st):
-            api.project(*calls(0))
+            call(0)
             torch.cuda.synchronize()
             with torch.cuda.graph(gr, stream=st):
                 for i in range(reps):
-                    api.project(*calls(i))
+                    call(i)
         torch.cuda.synchronize()
         return gr
 
+    # Nullspace::project as N (N^T b) from the nullity columns of N (the
+    # library's form) and, for comparison, as the product P b
     b = torch.from_numpy(x).cuda()
-    g = capture(lambda i: (P, b))
-    b.copy_(torch.from_numpy(x))
     replays = max(5, min(a.steps, 50))
     a_graph = argparse.Namespace(**{**vars(a), "steps": replays})
-    wall = _timed_steps(a_graph, g.replay)
-    per_call_ms = 1e3 * wall / (replays * reps)
-    ev = statistics.fmean(_events_time_ms(lambda: api.project(P, b), MIN_EVENT_REPS))
-    # the same calls with P cycled through copies of more than 256 MB in all,
-    # so that no call finds its P in the 256 MB Infinity Cache: the HBM rate
-    copies = int(np.ceil(300e6 / (P.numel() * P.element_size())))
-    Ps = [P.clone() for _ in range(copies)]
-    gc = capture(lambda i: (Ps[i % copies], b))
-    wall_c = _timed_steps(a_graph, gc.replay)
-    per_call_cycled_ms = 1e3 * wall_c / (replays * reps)
-    del gc, Ps
+
+    def graph_rate(call):
+        gr = capture(call)
+        b.copy_(torch.from_numpy(x))
+        w = _timed_steps(a_graph, gr.replay)
+        del gr
+        return w, 1e3 * w / (replays * reps)
+
+    wall, per_call_ms = graph_rate(lambda i: api.nullspace_project(N, rank, b))
+    ev = statistics.fmean(_events_time_ms(lambda: api.nullspace_project(N, rank, b), MIN_EVENT_REPS))
+    # the same calls with N cycled through copies whose read parts exceed
+    # 256 MB in all, so that no call finds its N in the Infinity Cache
+    nul = n - rank.to(torch.int64).clamp(max=n)
+    n_bytes = int(nul.sum()) * n * N.element_size()
+    copies = int(np.ceil(300e6 / n_bytes))
+    Ns = [N.clone() for _ in range(copies)]
+    _, per_call_cycled_ms = graph_rate(lambda i: api.nullspace_project(Ns[i % copies], rank, b))
+    del Ns
+    _, per_call_p_ms = graph_rate(lambda i: api.project(P, b))
+    copies_p = int(np.ceil(300e6 / (P.numel() * P.element_size())))
+    Ps = [P.clone() for _ in range(copies_p)]
+    _, per_call_p_cycled_ms = graph_rate(lambda i: api.project(Ps[i % copies_p], b))
+    del Ps
     # accuracy on the projected vectors of a fresh call
     b.copy_(torch.from_numpy(x))
-    api.project(P, b)
+    api.nullspace_project(N, rank, b)
     z = b.cpu().double().numpy()[:, 0, :]
     Am = np.swapaxes(A_t.astype(np.float64), 1, 2)
     res = np.linalg.norm(np.einsum("kij,kj->ki", Am, z), axis=1)
     rel = float(np.max(res / np.linalg.norm(x[:, 0, :].astype(np.float64), axis=1)))
-    alg = k * (n * n * 4 + 2 * n * 4)
+    alg = n_bytes + k * 2 * n * 4
+    alg_p = k * (n * n * 4 + 2 * n * 4)
     print(f"[bench] svd+nullspace build {tb:.3f} ms (svd {tsvd:.3f} ms, sweeps FP32 {s32.mean():.1f} + FP64 "
           f"{s64.mean():.1f}; all-FP64 sweeps {tsvd_64:.3f} ms), "
           f"{svd_roof['achieved']:.2f} TFLOP/s); project {per_call_ms * 1e3:.2f} us/call in graph "
-          f"({per_call_cycled_ms * 1e3:.2f} us with P cycled over {copies} copies), {ev * 1e3:.2f} us eager; "
-          f"max |A P x|/|x| {rel:.2e}", file=sys.stderr, flush=True)
+          f"({per_call_cycled_ms * 1e3:.2f} us with N cycled over {copies} copies), {ev * 1e3:.2f} us eager; "
+          f"as P b {per_call_p_ms * 1e3:.2f} / {per_call_p_cycled_ms * 1e3:.2f} us; max |A P x|/|x| {rel:.2e}",
+          file=sys.stderr, flush=True)
     cpu = None
     if not a.no_cpu_baseline:
         threads = _cpu_threads()
@@ -692,13 +706,20 @@ def bench_svd_nullspace(a):
             dt = time.perf_counter() - t1
             cpu = {"value": sample / dt, "unit": "nullspace builds/s", "cores": threads, "kind": "port",
                    "sample": f"SVD (one-sided Jacobi) + N + P of the first {sample} matrices in FP64, {dt:.2f}s"}
-    roof = _roofline("hbm", alg, per_call_cycled_ms, "gemv_quad_kernel",
-                     f"P read (96*96*4 B) + b read and written per matrix; calls cycle through {copies} copies of P "
-                     f"({copies * P.numel() * P.element_size() / 1e6:.0f} MB > the 256 MB Infinity Cache)")
+    roof = _roofline("hbm", alg, per_call_cycled_ms, "nullspace_project_kernel<float>",
+                     f"the nullity columns of N read (96 x (96 - rank) x 4 B, {n_bytes / 1e6:.1f} MB in all) + b read "
+                     f"and written per matrix; calls cycle through {copies} copies of N "
+                     f"({copies * n_bytes / 1e6:.0f} MB read > the 256 MB Infinity Cache)")
+    roof["via_P"] = {"kernel": "gemv_quad_kernel", "us_per_call": per_call_p_ms * 1e3,
+                     "us_per_call_cycled": per_call_p_cycled_ms * 1e3,
+                     "achieved_cycled": alg_p / (per_call_p_cycled_ms * 1e-3) / 1e9, "algorithmic": alg_p,
+                     "note": "the same projection as the product P b (96 x 96 P per matrix), P cycled over "
+                             f"{copies_p} copies"}
     roof["cache_resident"] = {"achieved": alg / (per_call_ms * 1e-3) / 1e9, "unit": "GB/s",
-                              "note": "one P (75.5 MB) for all 1000 calls: served in part from the Infinity "
-                                      "Cache, so not an HBM rate"}
-    return _line("Nullspace::project calls/s (2048 x 96x96 P, 1000 calls per HIP graph replay)",
+                              "note": "one N for all 1000 calls: served in part from the Infinity Cache, so "
+                                      "not an HBM rate"}
+    return _line("Nullspace::project calls/s (2048 projections onto 56-dim nullspaces in R^96, 1000 calls per "
+                 "HIP graph replay)",
                  1e3 / per_call_ms, "calls/s", a_graph, 1e3 * wall / replays, "f32",
                  {"workload": "svd_nullspace: k=2048 rank-40 48x96 FP32, sigma~U[0.1,10]; nullspace build then "
                               "1000 graph-captured project calls per step", "k": k, "m": m, "n": n,

@@ -449,3 +449,20 @@ def nullspace_batched(A: torch.Tensor, eps: float = 1e-6):
 def project(P: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
     """b_i <- P_i b_i in place (Nullspace::project: addAB with B aliasing C)."""
     return gemm_batched(P, b, b)
+
+
+def nullspace_project(N: torch.Tensor, rank: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+    """b_i <- N_i (N_i^T b_i) in place from the first n - rank_i columns of
+    N_i (as nullspace_batched returns N and rank): the projection P_i b_i
+    of Nullspace::project, reading N's nullity columns instead of P.
+    N (k, n, n), rank (k,) uint32/int32, b (k, 1, n)."""
+    _check_dev(N, "N")
+    _check_dev(b, "b")
+    k, n, n2 = N.shape
+    if n2 != n or b.shape != (k, 1, n) or b.dtype != N.dtype or rank.numel() != k:
+        raise GputilsError("nullspace_project: N (k, n, n), rank (k,), b (k, 1, n) of N's dtype")
+    if not b.is_contiguous() or not N.is_contiguous():
+        raise GputilsError("nullspace_project: contiguous tensors expected")
+    check(getattr(lib(), f"gputils_{_pfx(N)}nullspace_project_batched")(
+        n, _ptr(N), n, n * n, _ptr(rank), _ptr(b), n, k, stream_handle()), "nullspace_project_batched")
+    return b

@@ -951,6 +951,211 @@ int rank_batched(int n, const T *S, int64_t strideS, T eps, unsigned *rank, int6
     return launch_check("rank_kernel");
 }
 
+// b_i <- N_i (N_i^T b_i), N_i's first q_i = n - rank_i columns: the
+// projection P_i b_i of Nullspace::project (P = N N^T) formed from the
+// n x q_i block of N instead of the n x n P, so a call reads n q_i matrix
+// elements instead of n^2.  One workgroup per matrix; rank_i >= n gives
+// y = 0, as P = 0 does.
+//
+// Vector path (vec set by the host: n, ldn, strideN, strideB multiples of
+// V = 16 / sizeof(T) and 16-byte aligned N and b; taken when q_i <= KC G):
+// thread t owns the V-row slice r = t % (n/V) of the columns jg, jg + G, ...
+// (jg = t / (n/V), G = kThreads / (n/V)) and loads all of them, and its
+// slice of b, as 16-byte loads in flight together, so N is read from memory
+// once and held in registers for both products.  KC, the columns a thread
+// can hold, is the least of 4, 8, 12, 16 that covers q = n (16 when none
+// does; a larger q_i then takes the generic path).  z = N^T b: per-slice dot
+// products summed over the n/V slices through LDS; y = N z: per-thread
+// V-row partial sums added over the G column groups through LDS.
+//
+// Generic path (any n, ld, alignment): each 16-lane group takes one column
+// at a time for z (rows l + 16 k, a four-step DPP sum, b from LDS), then
+// S = 256 / n threads per row (a power of two) split the q columns of y and
+// add their partial sums through LDS, reading N a second time (from L2).
+// Word 3 of a raw (stride 0) buffer resource on gfx9: 32-bit data format,
+// default number format, no swizzle.
+constexpr int kBufferWord3 = 0x00020000;
+
+template <typename T, int KC>
+__device__ __forceinline__ void project_vec(int n, int q, const T *__restrict__ nm, int64_t ldn, T *bm,
+                                            unsigned char *lds) {
+    constexpr int V = 16 / sizeof(T);
+    typedef T vec __attribute__((ext_vector_type(V)));
+    const int tid = threadIdx.x, rq = n / V, G = kThreads / rq;
+    const int r = tid % rq, jg = tid / rq;
+    const bool act = jg < G;
+    T *part = reinterpret_cast<T *>(lds);  // rq x q dot products, later G x rq vec partial sums
+    vec *part2 = reinterpret_cast<vec *>(lds);
+    const size_t part_bytes = (size_t)rq * n * sizeof(T) > (size_t)kThreads * 16 ? (size_t)rq * n * sizeof(T)
+                                                                                    : (size_t)kThreads * 16;
+    T *zs = reinterpret_cast<T *>(lds + part_bytes);
+    // The q columns as one buffer resource: a load at or past column q (and
+    // every load of an idle thread, sent to column q) returns zeros without
+    // touching memory, so all KC loads issue unconditionally.
+    // (Base and size are workgroup-uniform; readfirstlane states it, so the
+    // resource lives in scalar registers and no waterfall loop is emitted.)
+    const uint64_t base = reinterpret_cast<uintptr_t>(nm);
+    const uint64_t ubase = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(base >> 32)) << 32) |
+                           (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)base);
+    const int bytes = __builtin_amdgcn_readfirstlane((int)((int64_t)q * ldn * (int64_t)sizeof(T)));
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void *>(ubase), 0, bytes, kBufferWord3);
+    vec nv[KC];
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+        const int j = act ? min(jg + u * G, q) : q;
+        nv[u] = __builtin_bit_cast(vec, __builtin_amdgcn_raw_buffer_load_b128(
+                                            rs, (int)(((int64_t)j * ldn + V * r) * (int64_t)sizeof(T)), 0, 0));
+    }
+    vec bv = {};
+    if (act) {
+        bv = *reinterpret_cast<const vec *>(bm + V * r);
+#pragma unroll
+        for (int u = 0; u < KC; ++u) {
+            const int j = jg + u * G;
+            if (j < q) {
+                const vec p = nv[u] * bv;
+                T d = p[0];
+#pragma unroll
+                for (int v = 1; v < V; ++v) d += p[v];
+                part[r * q + j] = d;
+            }
+        }
+    }
+    __syncthreads();
+    for (int j = tid; j < q; j += kThreads) {
+        T z0 = T(0), z1 = T(0);
+        int s = 0;
+        for (; s + 1 < rq; s += 2) {
+            z0 += part[s * q + j];
+            z1 += part[(s + 1) * q + j];
+        }
+        if (s < rq) z0 += part[s * q + j];
+        zs[j] = z0 + z1;
+    }
+    __syncthreads();
+    vec acc = {};
+    if (act) {
+#pragma unroll
+        for (int u = 0; u < KC; ++u) {
+            const int j = jg + u * G;
+            if (j < q) acc += nv[u] * zs[j];
+        }
+        part2[jg * rq + r] = acc;
+    }
+    __syncthreads();
+    if (tid < rq) {
+        vec y = part2[tid];
+        for (int g = 1; g < G; ++g) y += part2[g * rq + tid];
+        *reinterpret_cast<vec *>(bm + V * tid) = y;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void project_generic(int n, int q, const T *__restrict__ nm, int64_t ldn, T *bm, int S,
+                                                unsigned char *lds) {
+    T *bs = reinterpret_cast<T *>(lds);  // b (n)
+    T *zs = bs + n;                      // z (n >= q)
+    T *ps = zs + n;                      // y partial sums (S x n)
+    const int tid = threadIdx.x, g16 = tid >> 4, l16 = tid & 15;
+    for (int i = tid; i < n; i += kThreads) bs[i] = bm[i];
+    __syncthreads();
+    for (int c = g16; c < q; c += kThreads / 16) {
+        const T *col = nm + (int64_t)c * ldn;
+        T acc = T(0);
+#pragma unroll 4
+        for (int i = l16; i < n; i += 16) acc = fma(col[i], bs[i], acc);
+        acc = hh::row16_sum(acc);
+        if (l16 == 0) zs[c] = acc;
+    }
+    __syncthreads();
+    // row i, column slice h of S: columns c = h, h + S, ...
+    for (int e = tid; e < n * S; e += kThreads) {
+        const int i = e % n, h = e / n;
+        T a0 = T(0), a1 = T(0);
+        int c = h;
+        for (; c + S < q; c += 2 * S) {
+            a0 = fma(nm[i + (int64_t)c * ldn], zs[c], a0);
+            a1 = fma(nm[i + (int64_t)(c + S) * ldn], zs[c + S], a1);
+        }
+        if (c < q) a0 = fma(nm[i + (int64_t)c * ldn], zs[c], a0);
+        ps[h * n + i] = a0 + a1;
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += kThreads) {
+        T y = T(0);
+        for (int h = 0; h < S; ++h) y += ps[h * n + i];
+        bm[i] = y;
+    }
+}
+
+template <typename T, int KC>
+__global__ __launch_bounds__(kThreads) void nullspace_project_kernel(int n, const T *__restrict__ N, int64_t ldn,
+                                                                     int64_t strideN, const unsigned *__restrict__ rank,
+                                                                     T *b, int64_t strideB, int64_t batch, int S,
+                                                                     int vec) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char dyn_raw[];
+    const int64_t m = blockIdx.x + (int64_t)blockIdx.y * 65535;
+    if (m >= batch) return;  // uniform per workgroup
+    const unsigned rk = rank[m];
+    const int q = n - (int)(rk < (unsigned)n ? rk : (unsigned)n);
+    const T *nm = N + m * strideN;
+    T *bm = b + m * strideB;
+    if constexpr (KC > 0) {
+        if (vec && q <= KC * (kThreads / (n / (16 / (int)sizeof(T))))) {
+            project_vec<T, KC>(n, q, nm, ldn, bm, dyn_raw);
+            return;
+        }
+    }
+    project_generic<T>(n, q, nm, ldn, bm, S, dyn_raw);
+}
+
+template <typename T>
+int nullspace_project(int n, const T *N, int64_t ldn, int64_t strideN, const unsigned *rank, T *b, int64_t strideB,
+                      int64_t batch, void *stream) {
+    GPUTILS_ARG(n >= 1, 1, "n must be >= 1");
+    GPUTILS_ARG(ldn >= n, 3, "ldn < n");
+    GPUTILS_ARG(strideN >= 0, 4, "negative strideN");
+    GPUTILS_ARG(strideB >= 0, 7, "negative strideB");
+    GPUTILS_ARG(batch >= 0, 8, "negative batch");
+    if (batch == 0) return 0;
+    GPUTILS_ARG(N != nullptr, 2, "N is NULL");
+    GPUTILS_ARG(rank != nullptr, 5, "rank is NULL");
+    GPUTILS_ARG(b != nullptr, 6, "b is NULL");
+    int S = 1;
+    while (S * 2 * n <= kThreads) S *= 2;
+    const size_t generic_lds = sizeof(T) * (size_t)n * (2 + S);
+    GPUTILS_ARG(generic_lds <= (size_t)kMaxLds, 1, "n too large");
+    constexpr int V = 16 / sizeof(T);
+    const bool vec = n % V == 0 && n / V <= kThreads && ldn % V == 0 && strideN % V == 0 && strideB % V == 0 &&
+                     reinterpret_cast<uintptr_t>(N) % 16 == 0 && reinterpret_cast<uintptr_t>(b) % 16 == 0 &&
+                     (int64_t)n * ldn * (int64_t)sizeof(T) + 16 < ((int64_t)1 << 31);  // 32-bit buffer offsets
+    size_t lds = generic_lds;
+    if (vec) {
+        const size_t part = std::max((size_t)(n / V) * n * sizeof(T), (size_t)kThreads * 16);
+        lds = std::max(lds, part + sizeof(T) * (size_t)n);
+    }
+    GPUTILS_ARG(lds <= (size_t)kMaxLds, 1, "n too large");
+    int kc = 0;
+    if (vec) {
+        const int G = kThreads / (n / V), need = (n + G - 1) / G;
+        kc = need <= 4 ? 4 : need <= 8 ? 8 : need <= 12 ? 12 : 16;
+    }
+    auto kern = kc == 4    ? nullspace_project_kernel<T, 4>
+                : kc == 8  ? nullspace_project_kernel<T, 8>
+                : kc == 12 ? nullspace_project_kernel<T, 12>
+                : kc == 16 ? nullspace_project_kernel<T, 16>
+                           : nullspace_project_kernel<T, 0>;
+    if (lds > 64 * 1024)
+        GPUTILS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const unsigned gx = (unsigned)(batch < 65535 ? batch : 65535);
+    const unsigned gy = (unsigned)((batch + 65534) / 65535);
+    hipLaunchKernelGGL(kern, dim3(gx, gy), dim3(kThreads), lds, static_cast<hipStream_t>(stream), n, N, ldn, strideN,
+                       rank, b, strideB, batch, S, (int)vec);
+    return launch_check("nullspace_project_kernel");
+}
+
 template <typename T>
 int nullspace_build(int n, const T *U, int64_t ldu, int64_t strideU, const unsigned *rank, T *N, int64_t ldn,
                     int64_t strideN, T *P, int64_t ldp, int64_t strideP, int64_t batch, void *stream) {
@@ -1027,6 +1232,15 @@ int gputils_dnullspace_build_batched(int n, const double *U, int64_t ldu, int64_
                                      double *N, int64_t ldn, int64_t strideN, double *P, int64_t ldp,
                                      int64_t strideP, int64_t batch, void *stream) {
     return svd::nullspace_build<double>(n, U, ldu, strideU, rank, N, ldn, strideN, P, ldp, strideP, batch, stream);
+}
+
+int gputils_snullspace_project_batched(int n, const float *N, int64_t ldn, int64_t strideN, const unsigned *rank,
+                                       float *b, int64_t strideB, int64_t batch, void *stream) {
+    return svd::nullspace_project<float>(n, N, ldn, strideN, rank, b, strideB, batch, stream);
+}
+int gputils_dnullspace_project_batched(int n, const double *N, int64_t ldn, int64_t strideN, const unsigned *rank,
+                                       double *b, int64_t strideB, int64_t batch, void *stream) {
+    return svd::nullspace_project<double>(n, N, ldn, strideN, rank, b, strideB, batch, stream);
 }
 
 }  // extern "C"

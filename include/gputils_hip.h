@@ -353,6 +353,17 @@ int gputils_dnullspace_build_batched(int n, const double *U, int64_t ldu, int64_
                                      double *N, int64_t ldn, int64_t strideN, double *P, int64_t ldp,
                                      int64_t strideP, int64_t batch, void *stream);
 
+/* Nullspace::project (tensor.cuh:2083-2085, b_i <- P_i b_i with
+ * P_i = N_i N_i^T) formed as b_i <- N_i (N_i^T b_i) from the first
+ * n - rank[i] columns of N_i (as gputils_?nullspace_build_batched leaves
+ * them): the same projection up to rounding, reading n (n - rank[i])
+ * elements of N_i instead of the n^2 of P_i.  b_i: n contiguous elements,
+ * batch stride strideB; rank[i] >= n gives b_i = 0. */
+int gputils_snullspace_project_batched(int n, const float *N, int64_t ldn, int64_t strideN, const unsigned *rank,
+                                       float *b, int64_t strideB, int64_t batch, void *stream);
+int gputils_dnullspace_project_batched(int n, const double *N, int64_t ldn, int64_t strideN, const unsigned *rank,
+                                       double *b, int64_t strideB, int64_t batch, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

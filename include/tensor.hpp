@@ -1312,6 +1312,7 @@ class Nullspace {
 private:
     std::unique_ptr<DTensor<T>> m_nullspace;
     std::unique_ptr<DTensor<T>> m_projOp;
+    std::unique_ptr<DTensor<unsigned int>> m_rank;
 
 public:
     Nullspace(DTensor<T> &a) {
@@ -1328,7 +1329,8 @@ public:
         // R^T and U comes from the accumulated rotations (same subspaces)
         DTensor<T> S(m, 1, nMats), U(n, n, nMats);
         DTensor<int> info(1, 1, nMats);
-        DTensor<unsigned int> rank(1, 1, nMats, true);
+        m_rank = std::make_unique<DTensor<unsigned int>>(1, 1, nMats, true);
+        DTensor<unsigned int> &rank = *m_rank;
         gputils_detail::DeviceBuffer work((size_t)wb, false);
         void *st = aTranspose.stream();
         if constexpr (std::is_same_v<T, double>) {
@@ -1355,8 +1357,24 @@ public:
 
     DTensor<T> const &nullspace() const { return *m_nullspace; }
 
-    /** b_i <- P_i b_i in place (tensor.cuh:2081-2085). */
-    void project(DTensor<T> &b) { b.addAB(*m_projOp, b, 1, 0); }
+    /** b_i <- P_i b_i in place (tensor.cuh:2083-2085).  A single column per
+     * matrix is projected as N_i (N_i' b_i) from the nullity columns of N_i
+     * (fewer bytes read than P_i); several columns go through P. */
+    void project(DTensor<T> &b) {
+        const size_t n = m_nullspace->numRows();
+        if (b.numRows() == n && b.numCols() == 1 && b.numMats() == m_nullspace->numMats()) {
+            const int64_t nn = (int64_t)n * n, k = (int64_t)b.numMats();
+            if constexpr (std::is_same_v<T, double>) {
+                gpuErrChk(gputils_dnullspace_project_batched((int)n, m_nullspace->raw(), n, nn, m_rank->raw(),
+                                                             b.raw(), (int64_t)n, k, b.stream()));
+            } else {
+                gpuErrChk(gputils_snullspace_project_batched((int)n, m_nullspace->raw(), n, nn, m_rank->raw(),
+                                                             b.raw(), (int64_t)n, k, b.stream()));
+            }
+        } else {
+            b.addAB(*m_projOp, b, 1, 0);
+        }
+    }
 };
 
 /* ------------------------------------------------------------ Givens --- */

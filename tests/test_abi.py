@@ -64,6 +64,9 @@ def _msg():
     ("gputils_sgivens_annihilate", (4, 3, None, 4, 0, 1, 3, None), -7),
     ("gputils_gels_ragged_plan_describe", (None, None, 0, None), -1),
     ("gputils_gels_ragged_plan_tune", (None, None, None, 1, None), -1),
+    ("gputils_snullspace_project_batched", (0, None, 1, 0, None, None, 0, 1, None), -1),
+    ("gputils_dnullspace_project_batched", (8, None, 7, 64, None, None, 8, 1, None), -3),
+    ("gputils_snullspace_project_batched", (8, None, 8, 64, None, None, 8, 1, None), -2),  # N NULL
 ])
 def test_invalid_arguments_are_rejected(fn, args, code):
     assert getattr(_lib.lib(), fn)(*args) == code

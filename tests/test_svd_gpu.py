@@ -364,6 +364,76 @@ def test_config_project_1000_calls_in_one_graph():
         _ = (x0 * 2).sum().item()   # eager work between replays
 
 
+@pytest.mark.parametrize("m,n,rk", [(48, 96, 40), (3, 4, 2), (5, 5, 5), (20, 33, 7), (64, 130, 64), (10, 12, 0)])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_nullspace_project_from_n_equals_p(m, n, rk, dtype):
+    """Nullspace::project as N (N^T b) from N's nullity columns against
+    P b: the same projection to the type's rounding; A y = 0 for the
+    result; full rank (nullity 0) gives y = 0 exactly."""
+    k = 5
+    r = np.random.default_rng(10 * m + n + rk)
+    A = (np.einsum("kir,krj->kij", r.standard_normal((k, m, rk)), r.standard_normal((k, rk, n)))
+         if rk > 0 else np.zeros((k, m, n)))
+    tA = dev(api.from_matrices(torch.from_numpy(A)).numpy(), dtype)
+    N, P, rank = api.nullspace_batched(tA)
+    x = dev(r.standard_normal((k, 1, n)), dtype)
+    yp = api.project(P, x.clone())
+    yn = api.nullspace_project(N, rank, x.clone())
+    tol = 1e-5 if dtype == torch.float32 else 1e-12
+    scale = float(x.abs().max())
+    assert float((yp - yn).abs().max()) <= tol * scale
+    Z = yn.cpu().double().numpy()[:, 0, :]
+    for b in range(k):
+        assert np.linalg.norm(A[b] @ Z[b]) <= 10 * tol * max(np.linalg.norm(A[b]), 1.0) * scale * n
+        if int(rank[b]) >= n:
+            assert np.all(Z[b] == 0.0)
+
+
+def test_config_nullspace_project_1000_calls_in_one_graph():
+    """The configuration's projection through Nullspace::project's N form
+    at full size: 1000 calls captured in one HIP graph, replayed twice with
+    eager work between; each replay equals 1000 eager calls bit for bit and
+    ||A y|| <= 1e-5 ||x|| for every matrix."""
+    from gputils_amd import workloads
+    A_t, x = workloads.svd_nullspace()
+    N, P, rank = api.nullspace_batched(dev(A_t))
+    x0 = dev(x)
+    eager = x0.clone()
+    for _ in range(1000):
+        api.nullspace_project(N, rank, eager)
+    b = x0.clone()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        api.nullspace_project(N, rank, b)
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g, stream=s):
+            for _ in range(1000):
+                api.nullspace_project(N, rank, b)
+    torch.cuda.synchronize()
+    Am = api.to_matrices(torch.from_numpy(A_t)).double().numpy()
+    for rep in range(2):
+        b.copy_(x0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(b, eager), rep
+        z = b.cpu().double().numpy()[:, 0, :]
+        res = np.einsum("kij,kj->ki", Am, z)
+        assert np.max(np.linalg.norm(res, axis=1) / np.linalg.norm(x[:, 0, :], axis=1)) <= 1e-5
+        _ = (x0 * 2).sum().item()
+
+
+def test_nullspace_project_argument_checks():
+    N = torch.zeros((2, 4, 4), device="cuda")
+    rank = torch.zeros(2, dtype=torch.int32, device="cuda")
+    with pytest.raises(api.GputilsError):
+        api.nullspace_project(N, rank, torch.zeros((2, 1, 5), device="cuda"))
+    with pytest.raises(api.GputilsError):
+        api.nullspace_project(N, rank, torch.zeros((2, 1, 4), dtype=torch.float64, device="cuda"))
+
+
 def test_project_graph_replays_match_eager():
     """Nullspace::project captured in a HIP graph and replayed repeatedly,
     with eager work in between: every replay equals the eager result."""
