@@ -957,6 +957,9 @@ int rank_batched(int n, const T *S, int64_t strideS, T eps, unsigned *rank, int6
 // elements instead of n^2.  One workgroup per matrix; rank_i >= n gives
 // y = 0, as P = 0 does.
 //
+// Three paths, chosen by the host: the lane-group path below (n = 8 V RV,
+// RV <= 4), else the vector path, else the generic one.
+//
 // Vector path (vec set by the host: n, ldn, strideN, strideB multiples of
 // V = 16 / sizeof(T) and 16-byte aligned N and b; taken when q_i <= KC G):
 // thread t owns the V-row slice r = t % (n/V) of the columns jg, jg + G, ...
@@ -989,23 +992,26 @@ __device__ __forceinline__ void project_vec(int n, int q, const T *__restrict__ 
     const size_t part_bytes = (size_t)rq * n * sizeof(T) > (size_t)kThreads * 16 ? (size_t)rq * n * sizeof(T)
                                                                                     : (size_t)kThreads * 16;
     T *zs = reinterpret_cast<T *>(lds + part_bytes);
-    // The q columns as one buffer resource: a load at or past column q (and
-    // every load of an idle thread, sent to column q) returns zeros without
-    // touching memory, so all KC loads issue unconditionally.
-    // (Base and size are workgroup-uniform; readfirstlane states it, so the
-    // resource lives in scalar registers and no waterfall loop is emitted.)
+    // One buffer resource per held column slot u, based at column u G and
+    // sized to the q - u G columns left from there: a thread's load of
+    // column jg + u G is then the shared offset jg ldn + V r of resource u,
+    // and a load at or past column q (or any load of an idle thread, sent
+    // out of range) returns zeros without touching memory, so all KC loads
+    // issue unconditionally.  Base and size are workgroup-uniform;
+    // readfirstlane states it, so the resources live in scalar registers.
     const uint64_t base = reinterpret_cast<uintptr_t>(nm);
     const uint64_t ubase = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(base >> 32)) << 32) |
                            (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)base);
-    const int bytes = __builtin_amdgcn_readfirstlane((int)((int64_t)q * ldn * (int64_t)sizeof(T)));
-    const __amdgpu_buffer_rsrc_t rs =
-        __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void *>(ubase), 0, bytes, kBufferWord3);
+    const int uq = __builtin_amdgcn_readfirstlane(q);
+    const int off = act ? (int)(((int64_t)jg * ldn + V * r) * (int64_t)sizeof(T)) : 0x7ffffff0;
     vec nv[KC];
 #pragma unroll
     for (int u = 0; u < KC; ++u) {
-        const int j = act ? min(jg + u * G, q) : q;
-        nv[u] = __builtin_bit_cast(vec, __builtin_amdgcn_raw_buffer_load_b128(
-                                            rs, (int)(((int64_t)j * ldn + V * r) * (int64_t)sizeof(T)), 0, 0));
+        const int left = uq - u * G;
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            reinterpret_cast<void *>(ubase + (uint64_t)((int64_t)u * G * ldn * (int64_t)sizeof(T))), 0,
+            left > 0 ? (int)((int64_t)left * ldn * (int64_t)sizeof(T)) : 0, kBufferWord3);
+        nv[u] = __builtin_bit_cast(vec, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
     }
     vec bv = {};
     if (act) {
@@ -1089,7 +1095,87 @@ __device__ __forceinline__ void project_generic(int n, int q, const T *__restric
     }
 }
 
-template <typename T, int KC>
+// Lane-group path, n = 8 V RV (V = 16 / sizeof(T), RV = 1..4): each
+// column is read by one aligned group of 8 lanes, lane e taking rows
+// e RL ... e RL + RL - 1 (RL = V RV) as RV 16-byte loads; the 32 groups of
+// the workgroup take columns g, g + 32, ... (KS slots, all n covered), all
+// loaded at once through one range-checked buffer resource per slot, as in
+// the vector path.  z_j = (N^T b)_j is a three-step DPP sum within the
+// group, so z needs no LDS and no barrier.  y = N z: each lane sums its
+// rows over its KS columns; the two groups of a 16-lane row are added by a
+// row rotation, the 16 rows of the workgroup through LDS after the one
+// barrier (4 lanes per output vector, 4 rows each, then a DPP quad sum).
+template <typename T, int RV>
+__device__ __forceinline__ void project_lanes(int q, const T *__restrict__ nm, int64_t ldn, T *bm,
+                                              unsigned char *lds) {
+    constexpr int V = 16 / sizeof(T);
+    constexpr int n = 8 * V * RV;
+    constexpr int RL = V * RV;            // rows per lane
+    constexpr int CG = kThreads / 8;      // column groups
+    constexpr int KS = (n + CG - 1) / CG;  // column slots
+    constexpr int NV = n / V;             // vectors per column
+    typedef T vec __attribute__((ext_vector_type(V)));
+    const int tid = threadIdx.x, g = tid >> 3, e = tid & 7;
+    const uint64_t base = reinterpret_cast<uintptr_t>(nm);
+    const uint64_t ubase = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(base >> 32)) << 32) |
+                           (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)base);
+    const int uq = __builtin_amdgcn_readfirstlane(q);
+    const int off = (int)(((int64_t)g * ldn + e * RL) * (int64_t)sizeof(T));
+    vec nv[KS][RV];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        const int left = uq - s * CG;
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            reinterpret_cast<void *>(ubase + (uint64_t)((int64_t)s * CG * ldn * (int64_t)sizeof(T))), 0,
+            left > 0 ? (int)((int64_t)left * ldn * (int64_t)sizeof(T)) : 0, kBufferWord3);
+#pragma unroll
+        for (int v = 0; v < RV; ++v)
+            nv[s][v] = __builtin_bit_cast(vec, __builtin_amdgcn_raw_buffer_load_b128(rs, off + 16 * v, 0, 0));
+    }
+    vec bv[RV];
+#pragma unroll
+    for (int v = 0; v < RV; ++v) bv[v] = *reinterpret_cast<const vec *>(bm + e * RL + v * V);
+    vec acc[RV];
+#pragma unroll
+    for (int v = 0; v < RV; ++v) acc[v] = vec{};
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        vec p = nv[s][0] * bv[0];
+#pragma unroll
+        for (int v = 1; v < RV; ++v) p += nv[s][v] * bv[v];
+        T d = p[0];
+#pragma unroll
+        for (int c = 1; c < V; ++c) d += p[c];
+        d = hh::row16_plus<0xB1>(d);  // quad_perm [1,0,3,2]
+        d = hh::row16_plus<0x4E>(d);  // quad_perm [2,3,0,1]
+        d = hh::row16_plus<0x141>(d);  // row_half_mirror: the 8-lane sum
+#pragma unroll
+        for (int v = 0; v < RV; ++v) acc[v] += nv[s][v] * d;
+    }
+    vec *part = reinterpret_cast<vec *>(lds);  // 16 rows x NV vectors
+#pragma unroll
+    for (int v = 0; v < RV; ++v)
+#pragma unroll
+        for (int c = 0; c < V; ++c) acc[v][c] = hh::row16_plus<0x128>(acc[v][c]);  // row_ror:8
+    if ((tid & 8) == 0) {
+#pragma unroll
+        for (int v = 0; v < RV; ++v) part[(tid >> 4) * NV + e * RV + v] = acc[v];
+    }
+    __syncthreads();
+    if (tid < 4 * NV) {  // whole quads: NV vectors x 4 lanes
+        const int o = tid >> 2, h = tid & 3;
+        vec y = part[(4 * h) * NV + o] + part[(4 * h + 1) * NV + o] + part[(4 * h + 2) * NV + o] +
+                part[(4 * h + 3) * NV + o];
+#pragma unroll
+        for (int c = 0; c < V; ++c) {
+            y[c] = hh::row16_plus<0xB1>(y[c]);
+            y[c] = hh::row16_plus<0x4E>(y[c]);
+        }
+        if (h == 0) *reinterpret_cast<vec *>(bm + o * V) = y;
+    }
+}
+
+template <typename T, int KC, int RV>
 __global__ __launch_bounds__(kThreads) void nullspace_project_kernel(int n, const T *__restrict__ N, int64_t ldn,
                                                                      int64_t strideN, const unsigned *__restrict__ rank,
                                                                      T *b, int64_t strideB, int64_t batch, int S,
@@ -1101,6 +1187,10 @@ __global__ __launch_bounds__(kThreads) void nullspace_project_kernel(int n, cons
     const int q = n - (int)(rk < (unsigned)n ? rk : (unsigned)n);
     const T *nm = N + m * strideN;
     T *bm = b + m * strideB;
+    if constexpr (RV > 0) {
+        project_lanes<T, RV>(q, nm, ldn, bm, dyn_raw);
+        return;
+    }
     if constexpr (KC > 0) {
         if (vec && q <= KC * (kThreads / (n / (16 / (int)sizeof(T))))) {
             project_vec<T, KC>(n, q, nm, ldn, bm, dyn_raw);
@@ -1129,23 +1219,29 @@ int nullspace_project(int n, const T *N, int64_t ldn, int64_t strideN, const uns
     constexpr int V = 16 / sizeof(T);
     const bool vec = n % V == 0 && n / V <= kThreads && ldn % V == 0 && strideN % V == 0 && strideB % V == 0 &&
                      reinterpret_cast<uintptr_t>(N) % 16 == 0 && reinterpret_cast<uintptr_t>(b) % 16 == 0 &&
-                     (int64_t)n * ldn * (int64_t)sizeof(T) + 16 < ((int64_t)1 << 31);  // 32-bit buffer offsets
+                     (int64_t)(n + kThreads) * ldn * (int64_t)sizeof(T) < ((int64_t)1 << 31);  // 32-bit offsets
     size_t lds = generic_lds;
     if (vec) {
         const size_t part = std::max((size_t)(n / V) * n * sizeof(T), (size_t)kThreads * 16);
         lds = std::max(lds, part + sizeof(T) * (size_t)n);
+        lds = std::max(lds, (size_t)16 * 16 * (n / V));  // lane-group path: 16 rows of n / V vectors
     }
     GPUTILS_ARG(lds <= (size_t)kMaxLds, 1, "n too large");
-    int kc = 0;
+    int kc = 0, rv = 0;
     if (vec) {
         const int G = kThreads / (n / V), need = (n + G - 1) / G;
         kc = need <= 4 ? 4 : need <= 8 ? 8 : need <= 12 ? 12 : 16;
+        if (n % (8 * V) == 0 && n / (8 * V) <= 4) rv = n / (8 * V);
     }
-    auto kern = kc == 4    ? nullspace_project_kernel<T, 4>
-                : kc == 8  ? nullspace_project_kernel<T, 8>
-                : kc == 12 ? nullspace_project_kernel<T, 12>
-                : kc == 16 ? nullspace_project_kernel<T, 16>
-                           : nullspace_project_kernel<T, 0>;
+    auto kern = rv == 1    ? nullspace_project_kernel<T, 0, 1>
+                : rv == 2  ? nullspace_project_kernel<T, 0, 2>
+                : rv == 3  ? nullspace_project_kernel<T, 0, 3>
+                : rv == 4  ? nullspace_project_kernel<T, 0, 4>
+                : kc == 4  ? nullspace_project_kernel<T, 4, 0>
+                : kc == 8  ? nullspace_project_kernel<T, 8, 0>
+                : kc == 12 ? nullspace_project_kernel<T, 12, 0>
+                : kc == 16 ? nullspace_project_kernel<T, 16, 0>
+                           : nullspace_project_kernel<T, 0, 0>;
     if (lds > 64 * 1024)
         GPUTILS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

@@ -364,7 +364,9 @@ def test_config_project_1000_calls_in_one_graph():
         _ = (x0 * 2).sum().item()   # eager work between replays
 
 
-@pytest.mark.parametrize("m,n,rk", [(48, 96, 40), (3, 4, 2), (5, 5, 5), (20, 33, 7), (64, 130, 64), (10, 12, 0)])
+@pytest.mark.parametrize("m,n,rk", [(48, 96, 40), (3, 4, 2), (5, 5, 5), (20, 33, 7), (64, 130, 64), (10, 12, 0),
+                                    (16, 32, 16), (32, 32, 32), (30, 64, 20), (40, 48, 30), (120, 128, 100),
+                                    (96, 96, 90), (4, 96, 1)])
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 def test_nullspace_project_from_n_equals_p(m, n, rk, dtype):
     """Nullspace::project as N (N^T b) from N's nullity columns against
