@@ -706,7 +706,8 @@ def bench_svd_nullspace(a):
             dt = time.perf_counter() - t1
             cpu = {"value": sample / dt, "unit": "nullspace builds/s", "cores": threads, "kind": "port",
                    "sample": f"SVD (one-sided Jacobi) + N + P of the first {sample} matrices in FP64, {dt:.2f}s"}
-    roof = _roofline("hbm", alg, per_call_cycled_ms, "nullspace_project_kernel<float>",
+    # n = 96 FP32 takes the 8-lane column-group instantiation (3 loads per lane)
+    roof = _roofline("hbm", alg, per_call_cycled_ms, "nullspace_project_kernel<float,0,3>",
                      f"the nullity columns of N read (96 x (96 - rank) x 4 B, {n_bytes / 1e6:.1f} MB in all) + b read "
                      f"and written per matrix; calls cycle through {copies} copies of N "
                      f"({copies * n_bytes / 1e6:.0f} MB read > the 256 MB Infinity Cache)")
