@@ -19,7 +19,10 @@
 // U^T) reads W_I transposed from LDS and transposes the off-diagonal blocks
 // one at a time through LDS.  Refinement residuals r = b - A x are formed in
 // FP64 from the FP64 matrix in HBM, reading each 16x16 block of the lower
-// triangle once per residual; the iterate x stays FP64 in LDS.  Stopping
+// triangle once per residual; the iterate x stays FP64 in LDS, one element
+// per lane is read per block and the lanes that need it get it by DPP
+// broadcast, and each block's four right-hand-side sums are reduced over
+// the lane rows by one reduce-scatter.  Stopping
 // test per right-hand side (LAPACK DSPOSV):
 //   ||r||_inf <= tol (||A||_inf ||x||_inf + ||b||_inf).
 // Right-hand sides are processed four at a time (columns 0..3 of a 16x16
@@ -314,6 +317,59 @@ __device__ __forceinline__ double row_group_sum(double v) {
     return join(l32[0], h32[0]) + join(l32[1], h32[1]);
 }
 
+// Reduce-scatter of four values over the four 16-lane rows: row g
+// receives the sum over all rows of s[rs_rhs(g)], rs_rhs(g) = 2 (g & 1) +
+// (g >> 1).  v_permlane16_swap(s_k, s_k+2) leaves the pair sum of s_k in
+// the even rows and of s_k+2 in the odd rows (k = 0, 1); v_permlane32_swap
+// of those two then adds the row halves.  Six swaps and three adds, where
+// four full row sums take sixteen swaps and eight adds.
+__device__ __forceinline__ int rs_rhs(int g) { return ((g & 1) << 1) | (g >> 1); }
+__device__ __forceinline__ double row_group_reduce_scatter(const double (&s)[4]) {
+    auto lo = [](double v) { return (unsigned)(__double_as_longlong(v) & 0xffffffffll); };
+    auto hi = [](double v) { return (unsigned)(__double_as_longlong(v) >> 32); };
+    auto join = [](unsigned l, unsigned h) { return __longlong_as_double(((long long)h << 32) | l); };
+    double t[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const auto l16 = __builtin_amdgcn_permlane16_swap(lo(s[k]), lo(s[k + 2]), false, false);
+        const auto h16 = __builtin_amdgcn_permlane16_swap(hi(s[k]), hi(s[k + 2]), false, false);
+        t[k] = join(l16[0], h16[0]) + join(l16[1], h16[1]);
+    }
+    const auto l32 = __builtin_amdgcn_permlane32_swap(lo(t[0]), lo(t[1]), false, false);
+    const auto h32 = __builtin_amdgcn_permlane32_swap(hi(t[0]), hi(t[1]), false, false);
+    return join(l32[0], h32[0]) + join(l32[1], h32[1]);
+}
+
+// s[q] += x(lane 16 (lane / 16) + 4 r + q) a[r] for r, q < 4: sixteen FP64
+// FMAs whose first operand is x broadcast within the 16-lane row by DPP
+// (row_newbcast, the DPP control 64-bit arithmetic takes), so no separate
+// broadcast moves are issued (the compiler does not fold them into the
+// FMA).  Two wait states before (x may have just been written by the VALU:
+// a DPP source needs two) and after (the sums are read next by the lane
+// swaps of the reduction, which need two after a VALU write).
+__device__ __forceinline__ void fma_bcast16(double (&s)[4], const double (&a)[4], double x) {
+    asm("s_nop 1\n"
+        "v_fmac_f64_dpp %0, %4, %5 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %1, %4, %5 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %2, %4, %5 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %3, %4, %5 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %4, %6 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %1, %4, %6 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %2, %4, %6 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %3, %4, %6 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %4, %7 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %1, %4, %7 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %2, %4, %7 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %3, %4, %7 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %4, %8 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %1, %4, %8 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %2, %4, %8 row_newbcast:14 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %3, %4, %8 row_newbcast:15 row_mask:0xf bank_mask:0xf\n"
+        "s_nop 1"
+        : "+v"(s[0]), "+v"(s[1]), "+v"(s[2]), "+v"(s[3])
+        : "v"(x), "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]));
+}
+
 // Lower-triangle block t of the column-ordered sequence (0,0), (1,0), ...,
 // (NB-1,0), (1,1), ..., (NB-1,NB-1): (row block, column block).
 __host__ __device__ constexpr int sym_col(int t, int NB, int J = 0) {
@@ -381,7 +437,11 @@ __device__ __forceinline__ void residual_sym(const double *__restrict__ a, int64
                 if (m * 64 + 64 <= NP || lane + 64 * m < NP) ASUM[lane + 64 * m] = 0.0;
         }
     }
-    double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0, as = 0.0;
+    // x_K is read as one element per lane, XD[64 K + lane] (row 16 K + lane / 4,
+    // right-hand side lane % 4); lane (g, c) takes x(16 K + 4 g + r, q) from
+    // lane 16 g + 4 r + q of its row by DPP broadcast
+    const int qg = rs_rhs(g);
+    double p[4] = {0.0, 0.0, 0.0, 0.0}, as = 0.0;
     static_for<0, NT>([&](auto T) {
         constexpr int I = sym_row(T, NB), J = sym_col(T, NB);
         double av[4];
@@ -389,6 +449,7 @@ __device__ __forceinline__ void residual_sym(const double *__restrict__ a, int64
         for (int r = 0; r < 4; ++r) av[r] = ring[T % D][r];
         if constexpr (T + D < NT) sym_load<NB, T + D>(a, lda, ob, od, ring[T % D]);
         __builtin_amdgcn_sched_barrier(0);
+        const double xiv = XD[64 * I + lane];
         double at[4];
         if constexpr (I > J) {
             // at[r] = element (16I + 4g + r, 16J + c): the block transposed
@@ -397,23 +458,11 @@ __device__ __forceinline__ void residual_sym(const double *__restrict__ a, int64
 #pragma unroll
             for (int r = 0; r < 4; ++r) at[r] = TT[4 * g + r + 17 * c];
             // A(I,J) x_J into rows 16I + c
-            const double *xj = XD + 4 * (16 * J + 4 * g);
-            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const double *x = xj + 4 * r;
-                s0 += av[r] * x[0];
-                s1 += av[r] * x[1];
-                s2 += av[r] * x[2];
-                s3 += av[r] * x[3];
-            }
-            s0 = row_group_sum(s0);
-            s1 = row_group_sum(s1);
-            s2 = row_group_sum(s2);
-            s3 = row_group_sum(s3);
+            const double xjv = XD[64 * J + lane];
+            double s[4] = {0.0, 0.0, 0.0, 0.0};
+            fma_bcast16(s, av, xjv);
             const int o = 16 * I + c;
-            const double s01 = (g & 1) ? s1 : s0, s23 = (g & 1) ? s3 : s2;
-            RD[4 * o + g] -= (g & 2) ? s23 : s01;
+            RD[4 * o + qg] -= row_group_reduce_scatter(s);
             // |A| row sums, branch free (a branch per block here makes the
             // compiler keep the blocks live across it and spill): lane
             // group 0 adds to ASUM, the other groups to a sink
@@ -428,28 +477,15 @@ __device__ __forceinline__ void residual_sym(const double *__restrict__ a, int64
             for (int r = 0; r < 4; ++r) at[r] = av[r];
         }
         // A(I,J)^T x_I (the diagonal block itself when I = J) into rows 16J + c
-        const double *xi = XD + 4 * (16 * I + 4 * g);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const double *x = xi + 4 * r;
-            p0 += at[r] * x[0];
-            p1 += at[r] * x[1];
-            p2 += at[r] * x[2];
-            p3 += at[r] * x[3];
-        }
+        fma_bcast16(p, at, xiv);
         as += fabs(at[0]) + fabs(at[1]) + fabs(at[2]) + fabs(at[3]);
         if constexpr (I == NB - 1) {
-            p0 = row_group_sum(p0);
-            p1 = row_group_sum(p1);
-            p2 = row_group_sum(p2);
-            p3 = row_group_sum(p3);
             const int o = 16 * J + c;
-            const double p01 = (g & 1) ? p1 : p0, p23 = (g & 1) ? p3 : p2;
-            RD[4 * o + g] -= (g & 2) ? p23 : p01;
+            RD[4 * o + qg] -= row_group_reduce_scatter(p);
             as = row_group_sum(as);
             double *dst = (g == 0) ? ASUM + o : TT + kSinkOff + c;
             *dst = *dst + (want_asum ? as : 0.0);
-            p0 = p1 = p2 = p3 = as = 0.0;
+            p[0] = p[1] = p[2] = p[3] = as = 0.0;
         }
     });
 }
