@@ -403,10 +403,11 @@ __device__ __forceinline__ void sym_load(const double *__restrict__ a, int64_t l
 // A(I,J)^T x_I into the output rows of block J, accumulated per lane and
 // reduced over the lane groups once per column.  The block itself adds
 // A(I,J) x_J into the rows of block I, reduced at once and added to RD.
-template <int NB>
+template <int NB, bool WANT_ASUM>
 __device__ __forceinline__ void residual_sym(const double *__restrict__ a, int64_t lda, const double *XD,
                                              const double *__restrict__ bb, int64_t ldb, int nr, double *RD,
-                                             double *ASUM, bool want_asum, double *TT, int lane) {
+                                             double *ASUM, double *TT, int lane) {
+    constexpr bool want_asum = WANT_ASUM;
     constexpr int NP = 16 * NB, NT = NB * (NB + 1) / 2, D = 4;
     const int g = lane >> 4, c = lane & 15;
     uint32_t ob[4], od[4];
@@ -463,14 +464,13 @@ __device__ __forceinline__ void residual_sym(const double *__restrict__ a, int64
             fma_bcast16(s, av, xjv);
             const int o = 16 * I + c;
             RD[4 * o + qg] -= row_group_reduce_scatter(s);
-            // |A| row sums, branch free (a branch per block here makes the
-            // compiler keep the blocks live across it and spill): lane
-            // group 0 adds to ASUM, the other groups to a sink
-            {
+            // |A| row sums (first residual only): lane group 0 adds to
+            // ASUM, the other groups to a sink
+            if constexpr (WANT_ASUM) {
                 double sa = fabs(av[0]) + fabs(av[1]) + fabs(av[2]) + fabs(av[3]);
                 sa = row_group_sum(sa);
                 double *dst = (g == 0) ? ASUM + o : TT + kSinkOff + c;
-                *dst = *dst + (want_asum ? sa : 0.0);
+                *dst = *dst + sa;
             }
         } else {
 #pragma unroll
@@ -478,13 +478,15 @@ __device__ __forceinline__ void residual_sym(const double *__restrict__ a, int64
         }
         // A(I,J)^T x_I (the diagonal block itself when I = J) into rows 16J + c
         fma_bcast16(p, at, xiv);
-        as += fabs(at[0]) + fabs(at[1]) + fabs(at[2]) + fabs(at[3]);
+        if constexpr (WANT_ASUM) as += fabs(at[0]) + fabs(at[1]) + fabs(at[2]) + fabs(at[3]);
         if constexpr (I == NB - 1) {
             const int o = 16 * J + c;
             RD[4 * o + qg] -= row_group_reduce_scatter(p);
-            as = row_group_sum(as);
-            double *dst = (g == 0) ? ASUM + o : TT + kSinkOff + c;
-            *dst = *dst + (want_asum ? as : 0.0);
+            if constexpr (WANT_ASUM) {
+                as = row_group_sum(as);
+                double *dst = (g == 0) ? ASUM + o : TT + kSinkOff + c;
+                *dst = *dst + as;
+            }
             p[0] = p[1] = p[2] = p[3] = as = 0.0;
         }
     });
@@ -641,8 +643,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void po
             // orders the float accesses of the solve before the double
             // accesses of the residual to the same LDS words
             asm volatile("" ::: "memory");
-            if (n == 16 * NB && lda <= kSymMaxLda) residual_sym<NB>(a, lda, XD, bb, ldb, nr, RD, ASUM, anorm == 0.0, TS, ln);
-            else residual_padded<NB>(a, lda, n, XD, bb, ldb, nr, RD, ASUM, anorm == 0.0, ln);
+            if (n == 16 * NB && lda <= kSymMaxLda) {
+                // the |A| row sums are formed by the first residual only
+                if (anorm == 0.0) residual_sym<NB, true>(a, lda, XD, bb, ldb, nr, RD, ASUM, TS, ln);
+                else residual_sym<NB, false>(a, lda, XD, bb, ldb, nr, RD, ASUM, TS, ln);
+            } else {
+                residual_padded<NB>(a, lda, n, XD, bb, ldb, nr, RD, ASUM, anorm == 0.0, ln);
+            }
             if (anorm == 0.0) {
                 double m = 0.0;
 #pragma unroll
