@@ -283,19 +283,27 @@ __device__ __forceinline__ int factor(f4 (&U)[noff(NB)], float *DS) {
 }
 
 // Y <- U^{-T} Y, then Y <- U^{-1} Y (vector blocks).
+// The block products of one block row accumulate alternately into two
+// sums (two independent chains of MFMAs).
 template <int NB>
 __device__ __forceinline__ void solve(const f4 (&U)[noff(NB)], const float *DS, f4 (&Y)[NB], float *S, int lane) {
     static_for<0, NB>([&](auto Ic) {
         constexpr int I = Ic;
-        f4 s = zero4();
-        static_for<0, I>([&](auto Jc) { s = mtn(U[uo(Jc, I)], Y[Jc], s); });
-        Y[I] = mtn(lds_block(DS + 256 * I, lane), Y[I] - s, zero4());
+        f4 s0 = zero4(), s1 = zero4();
+        static_for<0, I>([&](auto Jc) {
+            if constexpr (Jc % 2 == 0) s0 = mtn(U[uo(Jc, I)], Y[Jc], s0);
+            else s1 = mtn(U[uo(Jc, I)], Y[Jc], s1);
+        });
+        Y[I] = mtn(lds_block(DS + 256 * I, lane), Y[I] - (s0 + s1), zero4());
     });
     static_for<0, NB>([&](auto tc) {
         constexpr int I = NB - 1 - tc;
-        f4 s = zero4();
-        static_for<I + 1, NB>([&](auto Kc) { s = mtn(transpose(U[uo(I, Kc)], S, lane), Y[Kc], s); });
-        Y[I] = mtn(lds_block_t(DS + 256 * I, lane), Y[I] - s, zero4());
+        f4 s0 = zero4(), s1 = zero4();
+        static_for<I + 1, NB>([&](auto Kc) {
+            if constexpr ((Kc - I) % 2 == 1) s0 = mtn(transpose(U[uo(I, Kc)], S, lane), Y[Kc], s0);
+            else s1 = mtn(transpose(U[uo(I, Kc)], S, lane), Y[Kc], s1);
+        });
+        Y[I] = mtn(lds_block_t(DS + 256 * I, lane), Y[I] - (s0 + s1), zero4());
     });
 }
 
