@@ -427,6 +427,65 @@ def test_config_nullspace_project_1000_calls_in_one_graph():
         _ = (x0 * 2).sum().item()
 
 
+def _project_reference(N, rank, b):
+    """y = N_q (N_q^T b) in float64 on the device, N_q the first n - rank
+    columns of each N (column j of matrix i is N[i, j, :])."""
+    k, n, _ = N.shape
+    q = (n - rank.to(torch.int64).clamp(max=n)).to(N.device)
+    keep = (torch.arange(n, device=N.device)[None, :] < q[:, None]).to(torch.float64)
+    Nd = N.to(torch.float64) * keep[:, :, None]
+    z = torch.bmm(Nd, b.to(torch.float64).transpose(1, 2).contiguous())       # (k, n, 1)
+    return torch.bmm(Nd.transpose(1, 2).contiguous(), z).transpose(1, 2)     # (k, 1, n)
+
+
+@pytest.mark.parametrize("n,dtype,k", [(32, torch.float32, 65540),   # lane groups, two grid rows
+                                       (64, torch.float64, 300),     # lane groups, FP64
+                                       (12, torch.float32, 300),     # vector path
+                                       (96, torch.float64, 300),     # vector path, FP64
+                                       (33, torch.float32, 300)])    # generic path
+def test_nullspace_project_paths_match_float64(n, dtype, k):
+    """Every path of the projection kernel (8-lane column groups, vector
+    slices, scalar) on random N and ranks from 0 to beyond n, against
+    float64 on the device: |y - y64| <= 4 u n max|N|^2 max|b| (u the type's
+    unit roundoff), and y = 0 exactly where rank >= n."""
+    g = torch.Generator(device="cuda").manual_seed(n * 7 + k)
+    N = torch.randn((k, n, n), generator=g, device="cuda", dtype=dtype)
+    rank = torch.randint(0, n + 3, (k,), generator=g, device="cuda", dtype=torch.int32)
+    b = torch.randn((k, 1, n), generator=g, device="cuda", dtype=dtype)
+    ref = _project_reference(N, rank, b)
+    y = api.nullspace_project(N, rank, b.clone())
+    u = torch.finfo(dtype).eps / 2
+    bound = 4 * u * n * float(N.abs().max()) ** 2 * float(b.abs().max()) * n
+    assert float((y.to(torch.float64) - ref).abs().max()) <= bound
+    full = rank.to(torch.int64) >= n
+    assert bool((y[full] == 0).all())
+
+
+def test_nullspace_project_unaligned_b_takes_the_scalar_path():
+    """b 4 bytes past a 16-byte boundary: the library falls back to the
+    scalar path and gives the same projection (checked against float64)."""
+    from gputils_amd import _lib
+    import ctypes
+    k, n = 257, 32
+    g = torch.Generator(device="cuda").manual_seed(5)
+    N = torch.randn((k, n, n), generator=g, device="cuda")
+    rank = torch.randint(0, n, (k,), generator=g, device="cuda", dtype=torch.int32)
+    buf = torch.zeros(k * n + 1, device="cuda")
+    b = buf[1:].view(k, 1, n)
+    b.copy_(torch.randn((k, 1, n), generator=g, device="cuda"))
+    ref = _project_reference(N, rank, b)
+    bmax = float(b.abs().max())
+    rc = _lib.lib().gputils_snullspace_project_batched(n, ctypes.c_void_p(N.data_ptr()), n, n * n,
+                                                       ctypes.c_void_p(rank.data_ptr()),
+                                                       ctypes.c_void_p(b.data_ptr()), n, k,
+                                                       ctypes.c_void_p(api.stream_handle()))
+    assert rc == 0
+    torch.cuda.synchronize()
+    bound = 4 * 2.0 ** -24 * n * float(N.abs().max()) ** 2 * bmax * n
+    assert float((b.to(torch.float64) - ref).abs().max()) <= bound
+    assert float(buf[0]) == 0.0   # nothing written before b
+
+
 def test_nullspace_project_argument_checks():
     N = torch.zeros((2, 4, 4), device="cuda")
     rank = torch.zeros(2, dtype=torch.int32, device="cuda")
