@@ -631,23 +631,31 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void po
         for (;;) {
             asm volatile("" ::: "memory");
             const int ln = opaque_lane();
-            const bool col_ok = (ln & 15) < kRhs;
+            // r into the vector-block layout: each lane reads one element
+            // per block, RD[64 I + lane], and item r of lane (g, c < 4),
+            // RD[64 I + 16 g + 4 r + c], comes from lane 16 g + 4 r + c of
+            // its row by a row shift of 4 r lanes written into the row's
+            // first four lanes only (DPP bank mask; the other lanes are the
+            // zero padding of the right-hand sides)
             f4 Y[NB];
-            {
-                const double *src = RD + 16 * (ln >> 4) + (ln & 3);
-                static_for<0, NB>([&](auto Ic) {
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) Y[Ic][r] = col_ok ? (float)src[64 * Ic + 4 * r] : 0.f;
-                });
-            }
+            static_for<0, NB>([&](auto Ic) {
+                const int v = __float_as_int((float)RD[64 * Ic + ln]);
+                Y[Ic][0] = __int_as_float(__builtin_amdgcn_update_dpp(0, v, 0xE4, 0xF, 0x1, false));   // own lane
+                Y[Ic][1] = __int_as_float(__builtin_amdgcn_update_dpp(0, v, 0x104, 0xF, 0x1, false));  // row_shl:4
+                Y[Ic][2] = __int_as_float(__builtin_amdgcn_update_dpp(0, v, 0x108, 0xF, 0x1, false));  // row_shl:8
+                Y[Ic][3] = __int_as_float(__builtin_amdgcn_update_dpp(0, v, 0x10C, 0xF, 0x1, false));  // row_shl:12
+            });
             solve<NB>(U, DS, Y, S, ln);
-            if (col_ok) {
-                double *dst = XD + 16 * (ln >> 4) + (ln & 3);
-                static_for<0, NB>([&](auto Ic) {
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) dst[64 * Ic + 4 * r] += (double)Y[Ic][r];
-                });
-            }
+            // x += d back in the element-per-lane layout: lane 16 g + 4 r + c
+            // takes item r of lane (g, c) by a row shift right of 4 r lanes
+            // written into bank r of the row
+            static_for<0, NB>([&](auto Ic) {
+                int y = __builtin_amdgcn_update_dpp(0, __float_as_int(Y[Ic][0]), 0xE4, 0xF, 0x1, false);
+                y = __builtin_amdgcn_update_dpp(y, __float_as_int(Y[Ic][1]), 0x114, 0xF, 0x2, false);  // row_shr:4
+                y = __builtin_amdgcn_update_dpp(y, __float_as_int(Y[Ic][2]), 0x118, 0xF, 0x4, false);  // row_shr:8
+                y = __builtin_amdgcn_update_dpp(y, __float_as_int(Y[Ic][3]), 0x11C, 0xF, 0x8, false);  // row_shr:12
+                XD[64 * Ic + ln] += (double)__int_as_float(y);
+            });
             // orders the float accesses of the solve before the double
             // accesses of the residual to the same LDS words
             asm volatile("" ::: "memory");
