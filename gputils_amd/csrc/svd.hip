@@ -18,9 +18,9 @@
 //      disjoint column pairs per round, 8 lanes per pair, one barrier per
 //      round; sweeps repeat until no pair needs a rotation,
 //      |w_p . w_q| <= n eps ||w_p|| ||w_q||.  Float data without VT
-//      (n <= 64) sweeps first in FP32 to the FP32 tolerance, then one
-//      Newton-Schulz step and W = W0 V in FP64 (fp32_first_sweeps), and
-//      the FP64 sweeps finish from there.
+//      (n <= 64) sweeps first in FP32 until the largest rotation left is
+//      below 1e-2, then one Newton-Schulz step and W = W0 V in FP64
+//      (fp32_first_sweeps), and the FP64 sweeps finish from there.
 //   3. sigma_j = ||w_j||, sorted descending; for nonzero sigma_j, v_j is
 //      signed so that its largest-magnitude entry is negative and
 //      u_j = w_j / sigma_j.  For a numerically zero sigma_j
@@ -284,15 +284,18 @@ __device__ void complete_basis_strided(double *X, int n, int nz, int lane) {
 }
 
 // Jacobi sweeps in the arithmetic of R on W (n x n, column-major, ld n),
-// the rotations accumulated in X, until a sweep rotates no pair or
-// max_sweeps have run; returns whether the last sweep rotated nothing.
-// Pair (p, q) is rotated when |w_p . w_q| > tol |w_p| |w_q| (squared: no
-// square roots on a round's critical path).  Parallel tournament ordering:
+// the rotations accumulated in X, until a sweep has no pair with
+// |w_p . w_q| > stop |w_p| |w_q| or max_sweeps have run; returns whether the
+// last sweep had none.  Pair (p, q) is rotated when |w_p . w_q| > tol |w_p|
+// |w_q| (squared: no square roots on a round's critical path).  stop = tol
+// runs to convergence; a larger stop ends once the largest rotation left is
+// below it (the first sweeps of a mixed-precision SVD: the FP64 sweeps that
+// follow finish the work).  Parallel tournament ordering:
 // n2/2 disjoint pairs per round, 8 lanes per pair, one barrier per round.
 // All threads of the block call it.
 // KPER: column entries per lane (n <= 8 KPER) in the register path.
 template <typename R, int MODE, int KPER>
-__device__ __forceinline__ bool jacobi_sweeps_k(R *W, R *X, int n, R tol2, int max_sweeps, int &sweeps) {
+__device__ __forceinline__ bool jacobi_sweeps_k(R *W, R *X, int n, R tol2, R stop2, int max_sweeps, int &sweeps) {
     const int tid = threadIdx.x;
     const int n2 = n + (n & 1), n1 = n2 - 1, npairs = n2 >> 1;
     const int grp = tid >> 3, gl = tid & 7;
@@ -364,6 +367,7 @@ __device__ __forceinline__ bool jacobi_sweeps_k(R *W, R *X, int n, R tol2, int m
                         be *= sc;
                         ga *= sc;
                     }
+                    if (ga * ga > stop2 * al * be) rot = 1;
                     if (ga * ga > tol2 * al * be) {
                         R cs, sn;
                         jacobi_angle(al, be, ga, cs, sn);
@@ -385,7 +389,6 @@ __device__ __forceinline__ bool jacobi_sweeps_k(R *W, R *X, int n, R tol2, int m
                                 vq[i] = sn * xv[u] + cs * yv[u];
                             }
                         }
-                        rot = 1;
                     }
                 }
             }
@@ -397,12 +400,12 @@ __device__ __forceinline__ bool jacobi_sweeps_k(R *W, R *X, int n, R tol2, int m
 }
 
 template <typename R, int MODE>
-__device__ __forceinline__ bool jacobi_sweeps(R *W, R *X, int n, R tol2, int max_sweeps, int &sweeps) {
+__device__ __forceinline__ bool jacobi_sweeps(R *W, R *X, int n, R tol2, R stop2, int max_sweeps, int &sweeps) {
     if constexpr (MODE != kWide) {
-        if (n <= 32) return jacobi_sweeps_k<R, MODE, 4>(W, X, n, tol2, max_sweeps, sweeps);
-        if (n <= 48) return jacobi_sweeps_k<R, MODE, 6>(W, X, n, tol2, max_sweeps, sweeps);
+        if (n <= 32) return jacobi_sweeps_k<R, MODE, 4>(W, X, n, tol2, stop2, max_sweeps, sweeps);
+        if (n <= 48) return jacobi_sweeps_k<R, MODE, 6>(W, X, n, tol2, stop2, max_sweeps, sweeps);
     }
-    return jacobi_sweeps_k<R, MODE, kMaxN / 8>(W, X, n, tol2, max_sweeps, sweeps);
+    return jacobi_sweeps_k<R, MODE, kMaxN / 8>(W, X, n, tol2, stop2, max_sweeps, sweeps);
 }
 
 // FP32 first sweeps for float data (n <= 64 held in LDS): on entry W holds
@@ -411,14 +414,14 @@ __device__ __forceinline__ bool jacobi_sweeps(R *W, R *X, int n, R tol2, int max
 // to FP64 level, ready for the FP64 sweeps.
 //   * W0 is scaled by a power of two to max |w| in [1, 2) and rounded to
 //     FP32 (W32) in the first half of X's space, X32 = I in the second;
-//     the FP32 sweeps run to the FP32 tolerance n u32 (rotations do not
-//     depend on the scale).
+//     the FP32 sweeps rotate the pairs above the FP32 tolerance n u32
+//     until a sweep's largest rotation measure is below 1e-2 (rotations
+//     do not depend on the scale).
 //   * X32 is orthogonal to about n u32.  One Newton-Schulz step
 //     X = X32 - F X32 / 2 with F = X32 X32^T - I (FP64 sums; F ~ 1e-6 is
 //     kept in FP32 over W32, which adds ~1e-13) brings that to ~1e-11.
 //   * W = W0 X in FP64, so that W = W0 X holds to FP64 rounding: the FP64
-//     sweeps then start from an off-diagonal level of ~u32 and need two or
-//     three sweeps.
+//     sweeps then finish in about three sweeps (the last one the check).
 // Lanes run over the row index i of each product, so one operand is read
 // along a column (consecutive addresses) and the other is the same for the
 // whole wave (broadcast).  Returns the FP32 sweep count (0 when max |W0| is
@@ -443,8 +446,12 @@ __device__ int fp32_first_sweeps(double *W, double *X, int n, bool lower, double
             X32[e] = (e % n == e / n) ? 1.0f : 0.0f;
         }
         __syncthreads();
+        // FP32 sweeps until the largest rotation measure |w_p . w_q| /
+        // (|w_p| |w_q|) of a sweep is below 1e-2: the FP64 sweeps that follow
+        // take it from there in as many sweeps as from the FP32-converged
+        // state (the last FP32 sweeps before convergence are saved)
         const float tol = (float)(n > 2 ? n : 2) * FLT_EPSILON;
-        jacobi_sweeps<float, MODE>(W32, X32, n, tol * tol, kMaxSweeps, s32);
+        jacobi_sweeps<float, MODE>(W32, X32, n, tol * tol, 1e-4f, kMaxSweeps, s32);
         float *F = W32;  // W32 is not needed any more
         for (int e = tid; e < nn; e += kThreads) {
             const int i = e % n, j = e / n;
@@ -580,7 +587,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
         if (n >= 4) s32 = fp32_first_sweeps<MODE>(W, X, n, on_rt, sig);  // sig[0..3]: wave maxima
     }
     const double tol = (double)(n > 2 ? n : 2) * DBL_EPSILON;
-    const bool converged = jacobi_sweeps<double, MODE>(W, X, n, tol * tol, kMaxSweeps, s64);
+    const bool converged = jacobi_sweeps<double, MODE>(W, X, n, tol * tol, tol * tol, kMaxSweeps, s64);
     __syncthreads();
 
     // 3. singular values, order, signs
