@@ -283,6 +283,71 @@ __device__ void complete_basis_strided(double *X, int n, int nz, int lane) {
     }
 }
 
+// Whether no column pair of W (n x n, n <= 64, column-major, ld n, FP64 in
+// LDS) needs a rotation: g_pq^2 <= tol2 g_pp g_qq for all p < q, with
+// G = W^T W.  This is the test a sweep that rotates nothing applies pair by
+// pair in n - 1 rounds; here G's upper tile triangle comes from FP64 MFMA
+// 16x16x4 products (wavefront w takes tiles w, w + 4, ..., the operands
+// read straight from W), its diagonal goes through gdiag (n doubles of
+// LDS), and one barrier-or gives the answer.
+__device__ bool gram_converged(const double *W, int n, double tol2, double *gdiag) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int T = (n + 15) >> 4, ntiles = T * (T + 1) / 2;
+    constexpr int kTilesPerWave = 3;  // 10 tiles for n = 64 over 4 wavefronts
+    f64x4 acc[kTilesPerWave];
+    int ta[kTilesPerWave], tb[kTilesPerWave];
+#pragma unroll
+    for (int u = 0; u < kTilesPerWave; ++u) {
+        const int t = wave + 4 * u;
+        acc[u] = Mfma16<double>::zero();
+        ta[u] = tb[u] = -1;
+        if (t < ntiles) {  // uniform per wavefront
+            int a = 0, rem = t;
+            while (rem >= T - a) {
+                rem -= T - a;
+                ++a;
+            }
+            const int bt = a + rem;
+            ta[u] = a;
+            tb[u] = bt;
+            const int p = 16 * a + (lane & 15), q = 16 * bt + (lane & 15);
+            const int pc = p < n ? p : n - 1, qc = q < n ? q : n - 1;
+            for (int k0 = 0; k0 < n; k0 += 4) {
+                const int i = k0 + (lane >> 4), ic = i < n ? i : n - 1;  // reads clamped into W
+                const double av = (i < n && p < n) ? W[ic + pc * n] : 0.0;
+                const double bv = (i < n && q < n) ? W[ic + qc * n] : 0.0;
+                acc[u] = Mfma16<double>::fma(av, bv, acc[u]);  // += W^T[p][i] W[i][q]
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kTilesPerWave; ++u) {
+        if (ta[u] >= 0 && ta[u] == tb[u]) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * ta[u] + Mfma16<double>::drow(lane, r), col = 16 * tb[u] + (lane & 15);
+                if (row == col && row < n) gdiag[row] = acc[u][r];
+            }
+        }
+    }
+    __syncthreads();
+    int bad = 0;
+#pragma unroll
+    for (int u = 0; u < kTilesPerWave; ++u) {
+        if (ta[u] >= 0) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * ta[u] + Mfma16<double>::drow(lane, r), col = 16 * tb[u] + (lane & 15);
+                if (row < col && col < n) {
+                    const double g = acc[u][r];
+                    if (g * g > tol2 * gdiag[row] * gdiag[col]) bad = 1;
+                }
+            }
+        }
+    }
+    return !__syncthreads_or(bad);
+}
+
 // Jacobi sweeps in the arithmetic of R on W (n x n, column-major, ld n),
 // the rotations accumulated in X, until a sweep has no pair with
 // |w_p . w_q| > stop |w_p| |w_q| or max_sweeps have run; returns whether the
@@ -290,12 +355,15 @@ __device__ void complete_basis_strided(double *X, int n, int nz, int lane) {
 // |w_q| (squared: no square roots on a round's critical path).  stop = tol
 // runs to convergence; a larger stop ends once the largest rotation left is
 // below it (the first sweeps of a mixed-precision SVD: the FP64 sweeps that
-// follow finish the work).  Parallel tournament ordering:
+// follow finish the work).  With gdiag (FP64, W in LDS) a sweep whose
+// rotations were all below 1e-4 is followed by gram_converged, which ends
+// the sweeps when a further sweep would rotate nothing.  Parallel tournament ordering:
 // n2/2 disjoint pairs per round, 8 lanes per pair, one barrier per round.
 // All threads of the block call it.
 // KPER: column entries per lane (n <= 8 KPER) in the register path.
 template <typename R, int MODE, int KPER>
-__device__ __forceinline__ bool jacobi_sweeps_k(R *W, R *X, int n, R tol2, R stop2, int max_sweeps, int &sweeps) {
+__device__ __forceinline__ bool jacobi_sweeps_k(R *W, R *X, int n, R tol2, R stop2, int max_sweeps, int &sweeps,
+                                                double *gdiag) {
     const int tid = threadIdx.x;
     const int n2 = n + (n & 1), n1 = n2 - 1, npairs = n2 >> 1;
     const int grp = tid >> 3, gl = tid & 7;
@@ -303,7 +371,7 @@ __device__ __forceinline__ bool jacobi_sweeps_k(R *W, R *X, int n, R tol2, R sto
     bool converged = (n1 == 0);
     while (!converged && sweeps < max_sweeps) {
         ++sweeps;
-        int rot = 0;
+        int rot = 0, far = 0;
         // players of pair grp in round 0 (tournament_pair), advanced by one
         // place per round: no division on the round's path
         int pa = (grp == 0) ? 0 : 1 + grp, pb = (grp == 0) ? 1 : 1 + n1 - grp;
@@ -368,6 +436,7 @@ __device__ __forceinline__ bool jacobi_sweeps_k(R *W, R *X, int n, R tol2, R sto
                         ga *= sc;
                     }
                     if (ga * ga > stop2 * al * be) rot = 1;
+                    if (ga * ga > R(1e-8) * al * be) far = 1;  // a rotation above 1e-4
                     if (ga * ga > tol2 * al * be) {
                         R cs, sn;
                         jacobi_angle(al, be, ga, cs, sn);
@@ -395,17 +464,25 @@ __device__ __forceinline__ bool jacobi_sweeps_k(R *W, R *X, int n, R tol2, R sto
             if (r + 1 < n1) __syncthreads();
         }
         converged = !__syncthreads_or(rot);
+        if constexpr (sizeof(R) == 8 && MODE != kWide) {
+            // once the largest rotation of a sweep is below 1e-4 the next
+            // sweep is likely to rotate nothing (quadratic convergence):
+            // test that on the Gram matrix instead of sweeping
+            if (!converged && gdiag != nullptr && !__syncthreads_or(far))
+                converged = gram_converged(W, n, tol2, gdiag);
+        }
     }
     return converged;
 }
 
 template <typename R, int MODE>
-__device__ __forceinline__ bool jacobi_sweeps(R *W, R *X, int n, R tol2, R stop2, int max_sweeps, int &sweeps) {
+__device__ __forceinline__ bool jacobi_sweeps(R *W, R *X, int n, R tol2, R stop2, int max_sweeps, int &sweeps,
+                                              double *gdiag = nullptr) {
     if constexpr (MODE != kWide) {
-        if (n <= 32) return jacobi_sweeps_k<R, MODE, 4>(W, X, n, tol2, stop2, max_sweeps, sweeps);
-        if (n <= 48) return jacobi_sweeps_k<R, MODE, 6>(W, X, n, tol2, stop2, max_sweeps, sweeps);
+        if (n <= 32) return jacobi_sweeps_k<R, MODE, 4>(W, X, n, tol2, stop2, max_sweeps, sweeps, gdiag);
+        if (n <= 48) return jacobi_sweeps_k<R, MODE, 6>(W, X, n, tol2, stop2, max_sweeps, sweeps, gdiag);
     }
-    return jacobi_sweeps_k<R, MODE, kMaxN / 8>(W, X, n, tol2, stop2, max_sweeps, sweeps);
+    return jacobi_sweeps_k<R, MODE, kMaxN / 8>(W, X, n, tol2, stop2, max_sweeps, sweeps, gdiag);
 }
 
 // FP32 first sweeps for float data (n <= 64 held in LDS): on entry W holds
@@ -421,7 +498,7 @@ __device__ __forceinline__ bool jacobi_sweeps(R *W, R *X, int n, R tol2, R stop2
 //     X = X32 - F X32 / 2 with F = X32 X32^T - I (FP64 sums; F ~ 1e-6 is
 //     kept in FP32 over W32, which adds ~1e-13) brings that to ~1e-11.
 //   * W = W0 X in FP64, so that W = W0 X holds to FP64 rounding: the FP64
-//     sweeps then finish in about three sweeps (the last one the check).
+//     sweeps then finish in about two sweeps and the Gram test.
 // Lanes run over the row index i of each product, so one operand is read
 // along a column (consecutive addresses) and the other is the same for the
 // whole wave (broadcast).  Returns the FP32 sweep count (0 when max |W0| is
@@ -587,7 +664,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
         if (n >= 4) s32 = fp32_first_sweeps<MODE>(W, X, n, on_rt, sig);  // sig[0..3]: wave maxima
     }
     const double tol = (double)(n > 2 ? n : 2) * DBL_EPSILON;
-    const bool converged = jacobi_sweeps<double, MODE>(W, X, n, tol * tol, tol * tol, kMaxSweeps, s64);
+    // sig (n doubles, free until the singular values) carries the Gram
+    // diagonal of the convergence test
+    const bool converged = jacobi_sweeps<double, MODE>(W, X, n, tol * tol, tol * tol, kMaxSweeps, s64, sig);
     __syncthreads();
 
     // 3. singular values, order, signs

@@ -314,7 +314,8 @@ int gputils_graph_destroy(gputils_graph_t graph);
  * both types; for float data with 4 <= n <= 64 and VT == NULL the sweeps
  * start in FP32 and FP64 sweeps finish them from the FP32 result
  * (gputils_gesvd_set_fp32_sweeps turns that off).  info[i] = 0, or 1 if the FP64 sweeps did not converge;
- * sweeps (may be NULL) receives the FP64 sweep count in bits 0..15 and the
+ * sweeps (may be NULL) receives the FP64 sweep count (a final convergence
+ * check made on W^T W instead of by a sweep is not counted) in bits 0..15 and the
  * FP32 one in bits 16..31.  work: gputils_gesvd_workspace_bytes(m, n, batch)
  * bytes of device memory (0 bytes, work may be NULL, when the matrix fits
  * in LDS).
