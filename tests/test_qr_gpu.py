@@ -332,6 +332,50 @@ def test_ragged_plan_graph_replays_match_eager(queue, concurrent):
         _ = (tA[:4096] * 2).sum().item()  # eager work between replays
 
 
+# edge shapes of the ragged plan: one column, one row, square systems, the
+# LDS bucket boundaries m n = 2048 / 4096 / 8192 / 16384 on both sides, the
+# largest system (512 x 64 = 32768, the last bucket's capacity) and a
+# consistent system (b in the range of A, zero residual)
+_EDGE_SHAPES = [(1, 1), (2, 1), (2, 2), (5, 5), (16, 1), (512, 1), (64, 64), (33, 32),
+                (512, 4), (455, 5), (512, 8), (293, 14), (512, 16), (257, 32), (512, 32),
+                (512, 33), (512, 64), (511, 64), (100, 64), (300, 7)]
+
+
+@pytest.mark.parametrize("method,waves", [(0, 2), (0, 16), (1, 8), (1, 16)])
+def test_ragged_edge_shapes_match_oracle(oracle, method, waves):
+    """Systems at the plan's size limits and bucket boundaries, one and two
+    refinement steps, against the FP64 oracle dgels, to the bounds of the
+    configuration tests (1e-9 and 1e-10 times cond(A)).  The last system is
+    consistent (b = A x): its solution must come out at the FP64 level
+    too.  Entries n..m-1 of b are left as they were."""
+    rng = np.random.default_rng(11)
+    shapes = _EDGE_SHAPES
+    ms = np.array([m for m, _ in shapes], dtype=np.int32)
+    ns = np.array([n for _, n in shapes], dtype=np.int32)
+    offA = np.concatenate([[0], np.cumsum(ms.astype(np.int64) * ns)[:-1]])
+    offB = np.concatenate([[0], np.cumsum(ms.astype(np.int64))[:-1]])
+    Af = rng.standard_normal(int((ms.astype(np.int64) * ns).sum()))
+    Bf = rng.standard_normal(int(ms.sum()))
+    # consistent last system: b = A x_true
+    m, n = shapes[-1]
+    A_last = Af[offA[-1]:offA[-1] + m * n].reshape(n, m).T
+    Bf[offB[-1]:offB[-1] + m] = A_last @ rng.standard_normal(n)
+    plan = api.RaggedLeastSquares(ms, ns, offA, offB)
+    for b in range(len(plan.describe())):
+        plan.configure(b, method, waves, 1)
+    tA = dev(Af)
+    for steps, bound in ((1, 1e-9), (2, 1e-10)):
+        out = _ragged_solve(plan, tA, Bf, steps)
+        for s, (m, n) in enumerate(shapes):
+            A = Af[offA[s]:offA[s] + m * n].reshape(n, m).T
+            b = Bf[offB[s]:offB[s] + m]
+            ref = oracle.gels(A, b).ravel()
+            x = out[offB[s]:offB[s] + n]
+            assert np.array_equal(out[offB[s] + n:offB[s] + m], b[n:]), (s, m, n)
+            err = np.linalg.norm(x - ref) / np.linalg.norm(ref)
+            assert err <= bound * max(np.linalg.cond(A), 1.0), (steps, s, m, n, err)
+
+
 def test_ragged_plan_configure_rejects_bad_shapes():
     from gputils_amd import workloads
     ms, ns, offA, offB, Af, Bf = workloads.ls_ragged(count=64)
