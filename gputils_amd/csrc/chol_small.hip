@@ -53,11 +53,27 @@ __device__ __forceinline__ void pin(T &x) { asm volatile("" : "+v"(x)); }
 __device__ __forceinline__ double rsq_estimate(double d) { return __builtin_amdgcn_rsq(d); }
 __device__ __forceinline__ float rsq_estimate(float d) { return __builtin_amdgcn_rsqf(d); }
 
+// Word 3 of a raw (stride 0) buffer resource: 32-bit data format, default
+// number format, no swizzle.  A buffer store at an offset at or past the
+// resource's size is dropped by the range check.
+constexpr int kBufferWord3 = 0x00020000;
+constexpr uint32_t kStoreDropped = 0x80000000u;
+
+__device__ __forceinline__ void buffer_store(__amdgpu_buffer_rsrc_t rs, uint32_t off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rs, (int)off, 0, 0);
+}
+__device__ __forceinline__ void buffer_store(__amdgpu_buffer_rsrc_t rs, uint32_t off, double v) {
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const u32x2 d = {(unsigned)b, (unsigned)(b >> 32)};
+    __builtin_amdgcn_raw_buffer_store_b64(d, rs, (int)off, 0, 0);
+}
+
 template <typename T, int N, bool WRITE_L, bool GIVEN_L>
 __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, T *A, int64_t lda, int64_t strideA,
                                                         const T *B, int64_t ldb, int64_t strideB, T *X,
                                                         int64_t ldx, int64_t strideX, int *info, int64_t batch,
-                                                        const int *mask) {
+                                                        const int *mask, uint32_t l_bytes) {
     constexpr int kTri = N * (N - 1) / 2;
     constexpr int G = 64 / N;  // matrices per wavefront, one per group of N lanes
     __shared__ __attribute__((aligned(16))) T LR[G][kTri];  // strict lower L, packed by rows
@@ -159,12 +175,37 @@ __global__ __launch_bounds__(64) void posv_small_kernel(int n, int nrhs, T *A, i
     T myri;
     if constexpr (GIVEN_L) myri = myri_given;
     else myri = ri[i < N ? i : 0];
-    if (WRITE_L && ok && i < n) {
-        const int iw = opaque(i);
-        static_for<0, N>([&](auto J) {
-            if (J < n && iw > J) a_g[i + (int64_t)J * lda] = a[J];
-        });
-        a_g[i + (int64_t)i * lda] = sd[i];
+    if constexpr (WRITE_L) {
+        if (l_bytes != 0) {
+            // one buffer resource over the wavefront's G matrices (l_bytes
+            // from the first one's base, < 2^31, checked by the host): every
+            // lane issues every store of its row, and a lane with no element
+            // of L there (right of the diagonal, a row past n, a matrix that
+            // failed or is not written) stores past the range, which the
+            // range check drops, so no store needs an exec mask of its own
+            const uint64_t base = reinterpret_cast<uintptr_t>(A + (int64_t)G * blockIdx.x * strideA);
+            const uint64_t ubase = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(base >> 32)) << 32) |
+                                   (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)base);
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                reinterpret_cast<void *>(ubase), 0, __builtin_amdgcn_readfirstlane((int)l_bytes), kBufferWord3);
+            // a storing lane's offsets row0 + J col (J <= i < n) lie inside
+            // the range; the others are replaced by the dropped offset
+            const bool lane_ok = ok & (i < n);
+            const uint32_t row0 = (uint32_t)(((int64_t)grp * strideA + i) * (int64_t)sizeof(T));
+            const uint32_t col = (uint32_t)(lda * (int64_t)sizeof(T));
+            const int iw = opaque(i);
+            static_for<0, N>([&](auto J) {
+                const bool st = lane_ok & (J < n) & (iw > J);
+                buffer_store(rs, st ? row0 + (uint32_t)J * col : kStoreDropped, a[J]);
+            });
+            buffer_store(rs, lane_ok ? row0 + (uint32_t)i * col : kStoreDropped, sd[i]);
+        } else if (ok && i < n) {
+            const int iw = opaque(i);
+            static_for<0, N>([&](auto J) {
+                if (J < n && iw > J) a_g[i + (int64_t)J * lda] = a[J];
+            });
+            a_g[i + (int64_t)i * lda] = sd[i];
+        }
     }
 
     const T *bb = B + (active ? b : 0) * strideB;
@@ -224,9 +265,17 @@ int launch_posv_small(int n, int nrhs, T *A, int64_t lda, int64_t strideA, const
     const int64_t per_wave = (n <= 16) ? 4 : 2;
     const int64_t blocks = (batch + per_wave - 1) / per_wave;
     const dim3 grid((unsigned)blocks), block(64);
+    // bytes a wavefront's buffer resource spans for the stores of L (from
+    // its first matrix to the last element of its last one); 0 when that
+    // is 2 GiB or more, or the strides are negative: then masked stores
+    const int64_t span = (per_wave - 1) * strideA + (int64_t)(n - 1) * lda + n;
+    const uint32_t l_bytes =
+        (write_l && strideA >= 0 && lda >= n && span * (int64_t)sizeof(T) < (int64_t)kStoreDropped)
+            ? (uint32_t)(span * (int64_t)sizeof(T))
+            : 0u;
 #define GPUTILS_SMALL(NV, WL)                                                                                        \
     hipLaunchKernelGGL((posv_small_kernel<T, NV, WL, false>), grid, block, 0, st, n, nrhs, A, lda, strideA, B, ldb, strideB, X, \
-                       ldx, strideX, info, batch, mask)
+                       ldx, strideX, info, batch, mask, l_bytes)
     if (n <= 16) {
         if (write_l) GPUTILS_SMALL(16, true);
         else GPUTILS_SMALL(16, false);
@@ -248,10 +297,10 @@ int launch_potrs_small(int n, int nrhs, const T *L, int64_t lda, int64_t strideL
     T *l = const_cast<T *>(L);
     if (n <= 16)
         hipLaunchKernelGGL((posv_small_kernel<T, 16, false, true>), grid, block, 0, st, n, nrhs, l, lda, strideL, B, ldb,
-                           strideB, B, ldb, strideB, info, batch, nullptr);
+                           strideB, B, ldb, strideB, info, batch, nullptr, 0u);
     else
         hipLaunchKernelGGL((posv_small_kernel<T, 32, false, true>), grid, block, 0, st, n, nrhs, l, lda, strideL, B, ldb,
-                           strideB, B, ldb, strideB, info, batch, nullptr);
+                           strideB, B, ldb, strideB, info, batch, nullptr, 0u);
     return launch_check("posv_small_kernel (solve only)");
 }
 

@@ -457,6 +457,40 @@ def test_small_fused_posv_padded_leading_dimensions(oracle, n, nrhs):
     assert np.all(Bo[:, :nrhs, n:] == -3.5) and np.all(Bo[:, nrhs, :] == -3.5)
 
 
+@pytest.mark.parametrize("stride", [(1 << 28) - 1024, 1 << 28])
+def test_small_fused_posv_batch_stride_near_2_gib(oracle, stride):
+    """Two 20x20 systems (one wavefront) whose batch stride puts the second
+    one about 2 GiB after the first: just below 2^31 bytes the factor is
+    stored through the wavefront's range-checked buffer resource (offsets
+    up to ~2^31), at 2^31 and beyond through per-lane masked stores.  Both
+    give the oracle's L and x and leave the upper triangles and the bytes
+    between the matrices as they were."""
+    n, nrhs, k = 20, 1, 2
+    A = spd_batch(k, n, seed=77)
+    B = np.random.default_rng(78).standard_normal((k, nrhs, n))
+    big = torch.full((stride + n * n,), 1.5, dtype=torch.float64, device="cuda")
+    for b in range(k):
+        big[b * stride:b * stride + n * n] = torch.from_numpy(np.ascontiguousarray(A[b].T).ravel()).cuda()
+    tB = dev(B)
+    info = torch.empty(k, dtype=torch.int32, device="cuda")
+    rc = api.lib().gputils_dposv_batched(n, nrhs, api._ptr(big), n, stride, api._ptr(tB), n, n * nrhs,
+                                         api._ptr(info), k, api.stream_handle())
+    assert rc == 0
+    torch.cuda.synchronize()
+    assert info.cpu().tolist() == [0] * k
+    Bo = tB.cpu().numpy()
+    for b in range(k):
+        Lg = big[b * stride:b * stride + n * n].cpu().numpy().reshape(n, n).T
+        Lo, _ = oracle.potrf(A[b])
+        assert np.max(np.abs(np.tril(Lg) - np.tril(Lo))) <= 1e-12 * np.abs(Lo).max()
+        assert np.array_equal(np.triu(Lg, 1), np.triu(A[b], 1))
+        ref = oracle.potrs(Lo, B[b].T)
+        assert np.max(np.abs(Bo[b].T - ref)) <= 1e-11 * np.abs(ref).max()
+    gap = big[n * n:stride]
+    assert bool((gap == 1.5).all())
+    del big
+
+
 def test_mixed_fallback_many_systems_per_workgroup():
     """The FP64 fallback launch covers the batch with 512 workgroups, so at
     k = 1100 a workgroup owns up to three systems (b, b + 512, b + 1024).
