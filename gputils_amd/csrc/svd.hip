@@ -951,6 +951,85 @@ __global__ __launch_bounds__(kThreads) void nullspace_build_kernel(int n, const 
     }
 }
 
+// The same outputs with P = N N^T on the matrix cores: one workgroup per
+// matrix, the nullity columns of U staged once in LDS (np = 16 ceil(n / 16)
+// rows and kp = 4 ceil(q / 4) columns, the padding zero, so every operand
+// read is unconditional) while N is written; wavefront w then computes the
+// 16 x 16 tiles w, w + 4, ... of P, each by q / 4 16x16x4 MFMAs whose A and B
+// operands are rows of the staged block (A[i][k] = N[16 ti + i][k],
+// B[k][j] = N[16 tj + j][k]).  Every tile of P is formed (the MFMA work is
+// small next to the writes of N and P), stored with one 16-byte store per
+// lane when the rows of a lane are consecutive and aligned (FP32, ldp and
+// strideP multiples of 4), else element by element.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void nullspace_build_mfma_kernel(int n, const T *__restrict__ U, int64_t ldu,
+                                                                        int64_t strideU, const unsigned *rank, T *N,
+                                                                        int64_t ldn, int64_t strideN, T *P,
+                                                                        int64_t ldp, int64_t strideP, int64_t batch,
+                                                                        int vec4) {
+    extern __shared__ unsigned char dyn_raw[];
+    T *Ns = reinterpret_cast<T *>(dyn_raw);  // kp columns of np rows, ld np
+    const int64_t b = blockIdx.x;
+    if (b >= batch) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const unsigned rk = rank[b];
+    const int r = (int)(rk < (unsigned)n ? rk : (unsigned)n);
+    const int nul = n - r;
+    const int nt = (n + 15) >> 4, np = 16 * nt, kp = (nul + 3) & ~3;
+    const int ncol = n > kp ? n : kp;
+    const T *ub = U + b * strideU;
+    T *nb = N + b * strideN;
+    T *pb = P + b * strideP;
+    // eight elements per thread at a time, their loads issued together.
+    // Column of element e: floor((e + 1/2) / np) in FP32, exact for
+    // e < 2^16 and np a multiple of 16 up to 512 (the LDS bound keeps
+    // np ncol below 2^16): (e + 1/2) / np lies at least 1 / (2 np) from an
+    // integer and the float quotient within 2^-17 of it.
+    constexpr int kU = 8;
+    const int total = np * ncol;
+    const float inv_np = 1.0f / (float)np;
+    auto col_of = [&](int e) { return (int)(((float)e + 0.5f) * inv_np); };
+    for (int e0 = tid; e0 < total; e0 += kU * kThreads) {
+        T v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int e = e0 + u * kThreads, c = col_of(e), i = e - c * np;
+            v[u] = (e < total && i < n && c < nul) ? ub[i + (int64_t)(r + c) * ldu] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int e = e0 + u * kThreads, c = col_of(e), i = e - c * np;
+            if (e < total) {
+                if (i < n && c < n) nb[i + (int64_t)c * ldn] = v[u];
+                if (c < kp) Ns[e] = v[u];
+            }
+        }
+    }
+    __syncthreads();
+    using M = Mfma16<T>;
+    const int col_in_tile = lane & 15, kq = lane >> 4;
+    for (int t = wave; t < nt * nt; t += kThreads / kWave) {
+        const int ti = t / nt, tj = t % nt;
+        const T *as = Ns + 16 * ti + col_in_tile + kq * np;
+        const T *bs = Ns + 16 * tj + col_in_tile + kq * np;
+        typename M::acc_t acc = M::zero();
+        for (int k0 = 0; k0 < kp; k0 += 4) acc = M::fma(as[(int64_t)k0 * np], bs[(int64_t)k0 * np], acc);
+        const int col = 16 * tj + col_in_tile;
+        if constexpr (sizeof(T) == 4) {
+            const int row0 = 16 * ti + 4 * kq;  // rows row0 .. row0 + 3 (f32 layout)
+            if (vec4 && row0 + 3 < n && col < n) {
+                *reinterpret_cast<f32x4 *>(pb + row0 + (int64_t)col * ldp) = acc;
+                continue;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int row = 16 * ti + M::drow(lane, q);
+            if (row < n && col < n) pb[row + (int64_t)col * ldp] = acc[q];
+        }
+    }
+}
+
 // Where a matrix of m x n lives: in LDS when it fits, G in the workspace
 // next, everything in the workspace for n > kMaxN or very tall matrices.
 inline int gesvd_mode(int m, int n) {
@@ -1354,6 +1433,22 @@ int nullspace_build(int n, const T *U, int64_t ldu, int64_t strideU, const unsig
     GPUTILS_ARG(rank != nullptr, 5, "rank is NULL");
     GPUTILS_ARG(N != nullptr, 6, "N is NULL");
     GPUTILS_ARG(P != nullptr, 9, "P is NULL");
+    // the matrix-core form when its staged block fits the LDS
+    const int np = 16 * ((n + 15) >> 4), kp = (n + 3) & ~3;
+    const size_t lds_mfma = (size_t)np * (size_t)(n > kp ? n : kp) * sizeof(T);
+    if (lds_mfma <= (size_t)kMaxLds) {
+        auto kern = nullspace_build_mfma_kernel<T>;
+        if (lds_mfma > 64 * 1024)
+            GPUTILS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_mfma));
+        const int vec4 = (sizeof(T) == 4 && ldp % 4 == 0 && strideP % 4 == 0 &&
+                          reinterpret_cast<uintptr_t>(P) % 16 == 0)
+                             ? 1
+                             : 0;
+        hipLaunchKernelGGL(kern, dim3((unsigned)batch), dim3(kThreads), lds_mfma, static_cast<hipStream_t>(stream), n,
+                           U, ldu, strideU, rank, N, ldn, strideN, P, ldp, strideP, batch, vec4);
+        return launch_check("nullspace_build_mfma_kernel");
+    }
     const size_t lds = (size_t)kNbKc * n * sizeof(T);
     GPUTILS_ARG(lds <= (size_t)kMaxLds, 1, "n too large");
     auto kern = nullspace_build_kernel<T>;
