@@ -205,9 +205,66 @@ __global__ __launch_bounds__(kThreads) void transpose_kernel(int m, int n, const
     }
 }
 
+// Small matrices with no padding (lda = m, ldb = n): one workgroup per
+// matrix, the whole matrix through LDS.  A is read and B written as the
+// contiguous blocks they are (consecutive lanes on consecutive elements);
+// the LDS copy holds B with the odd column stride n + 1, so both the
+// transposing writes (stride n + 1 across lanes) and the reads are
+// conflict-free.  Eight loads per thread are issued together.  Element e
+// of a column-major p x q block sits in column floor((e + 1/2) / p),
+// computed in FP32: exact for e < 2^16 and p <= 512, since (e + 1/2) / p
+// lies at least 1 / (2p) from an integer and the float quotient within
+// 2^-17 of it.
+constexpr int kSmallTransposeElems = 8192;
+constexpr int kSmallTransposeMaxDim = 512;
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void transpose_small_kernel(int m, int n, const T *__restrict__ A,
+                                                                   int64_t strideA, T *__restrict__ B,
+                                                                   int64_t strideB) {
+    extern __shared__ unsigned char dyn_raw[];
+    T *S = reinterpret_cast<T *>(dyn_raw);  // B (n x m) with column stride n + 1
+    const T *a = A + (int64_t)blockIdx.x * strideA;
+    T *bp = B + (int64_t)blockIdx.x * strideB;
+    const int total = m * n, lds = n + 1;
+    const float inv_m = 1.0f / (float)m, inv_n = 1.0f / (float)n;
+    constexpr int kU = 8;
+    for (int e0 = threadIdx.x; e0 < total; e0 += kU * kThreads) {
+        T v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int e = e0 + u * kThreads;
+            v[u] = a[e < total ? e : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int e = e0 + u * kThreads;
+            const int j = (int)(((float)e + 0.5f) * inv_m), i = e - j * m;  // A(i, j)
+            if (e < total) S[j + i * lds] = v[u];
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < total; e += kThreads) {
+        const int c = (int)(((float)e + 0.5f) * inv_n), r = e - c * n;  // B(r, c)
+        bp[e] = S[r + c * lds];
+    }
+}
+
 template <typename T>
 int transpose(int64_t m, int64_t n, const T *A, int64_t lda, int64_t strideA, T *B, int64_t ldb,
               int64_t strideB, int64_t batch, hipStream_t st) {
+    const size_t lds = (size_t)(n + 1) * (size_t)m * sizeof(T);
+    if (lda == m && ldb == n && m * n <= kSmallTransposeElems && m <= kSmallTransposeMaxDim &&
+        n <= kSmallTransposeMaxDim && lds <= 64 * 1024) {
+        for (int64_t b0 = 0; b0 < batch; b0 += 0x7fffffff) {
+            const int64_t nb = (batch - b0 < 0x7fffffff) ? (batch - b0) : 0x7fffffff;
+            hipLaunchKernelGGL((transpose_small_kernel<T>), dim3((unsigned)nb), dim3(kThreads), lds, st, (int)m,
+                               (int)n, A + b0 * strideA, strideA, B + b0 * strideB, strideB);
+            int rc = launch_check("transpose_small_kernel");
+            if (rc) return rc;
+        }
+        return GPUTILS_OK;
+    }
     const int tiles_m = (int)ceil_div(m, 32), tiles_n = (int)ceil_div(n, 32);
     for (int64_t b0 = 0; b0 < batch; b0 += 65535) {
         const int64_t nb = (batch - b0 < 65535) ? (batch - b0) : 65535;

@@ -175,3 +175,9 @@ def test_transpose_golden_and_random(dtype):
     T = api.transpose_batched(dev(X, dtype)).cpu().double().numpy()
     ref = np.swapaxes(X.astype(np.float32) if dtype == torch.float32 else X, 1, 2)
     assert np.array_equal(T, ref)
+    # larger than the whole-matrix-in-LDS form (tiled kernel), and odd sizes
+    for shape in ((3, 300, 200), (5, 7, 513), (4, 33, 17), (2, 1, 1), (3, 1, 40)):
+        X = r.standard_normal(shape)
+        T = api.transpose_batched(dev(X, dtype)).cpu().double().numpy()
+        ref = np.swapaxes(X.astype(np.float32) if dtype == torch.float32 else X, 1, 2)
+        assert np.array_equal(T, ref), shape
