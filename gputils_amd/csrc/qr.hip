@@ -404,16 +404,18 @@ __device__ __forceinline__ void apply_q_regs(const float *S, int m, int n, const
 }
 
 // Solve R y = c in place (R the n x n upper triangle of S, ld m; c in LDS,
-// n <= 64), one wavefront: lane i holds c_i, column k - 1 of R is read while
-// step k runs.
+// n <= 64), one wavefront: lane i holds c_i and 1 / R(i, i) (formed for all
+// rows at once, so that a step multiplies instead of dividing on the
+// chain), column k - 1 of R is read while step k runs.
 __device__ __forceinline__ void trsv_upper_regs(const float *S, int m, int n, float *c) {
     const int lane = threadIdx.x & 63;
     float cv = (lane < n) ? c[lane] : 0.0f;
+    const float dinv = (lane < n) ? 1.0f / S[lane + (int64_t)lane * m] : 0.0f;
     float rk = (lane < n) ? S[lane + (int64_t)(n - 1) * m] : 0.0f;
 #pragma unroll 1
     for (int k = n - 1; k >= 0; --k) {
         const float rn = (lane < k) ? S[lane + (int64_t)(k - 1) * m] : 0.0f;  // rows 0..k-1 of column k-1
-        const float ck = readlane(cv, k) / readlane(rk, k);
+        const float ck = readlane(cv, k) * readlane(dinv, k);
         cv = (lane == k) ? ck : ((lane < k) ? fmaf(-rk, ck, cv) : cv);
         rk = rn;
     }
@@ -421,15 +423,16 @@ __device__ __forceinline__ void trsv_upper_regs(const float *S, int m, int n, fl
 }
 
 // Solve R^T y = c in place (forward substitution), one wavefront: lane i
-// holds c_i, row k + 1 of R is read while step k runs.
+// holds c_i and 1 / R(i, i), row k + 1 of R is read while step k runs.
 __device__ __forceinline__ void trsv_upper_trans_regs(const float *S, int m, int n, float *c) {
     const int lane = threadIdx.x & 63;
     float cv = (lane < n) ? c[lane] : 0.0f;
+    const float dinv = (lane < n) ? 1.0f / S[lane + (int64_t)lane * m] : 0.0f;
     float rk = (lane < n) ? S[(int64_t)lane * m] : 0.0f;  // row 0
 #pragma unroll 1
     for (int k = 0; k < n; ++k) {
         const float rn = (lane > k && lane < n) ? S[(k + 1) + (int64_t)lane * m] : 0.0f;  // row k+1, columns > k
-        const float ck = readlane(cv, k) / readlane(rk, k);
+        const float ck = readlane(cv, k) * readlane(dinv, k);
         cv = (lane == k) ? ck : ((lane > k) ? fmaf(-rk, ck, cv) : cv);
         rk = rn;
     }
